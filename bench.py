@@ -1,0 +1,176 @@
+#!/usr/bin/env python3
+"""Benchmark: logL + branch-length gradient site-pattern evaluations per
+second on one MI355X.
+
+Workload (default): 100-taxon random rooted binary tree, branch lengths
+~ Exp(mean 0.05), 4-state GTR (exchangeabilities ~ Dirichlet(1), base
+frequencies ~ Dirichlet(5)), 4 discrete-gamma categories (alpha = 0.5),
+10,000 distinct simulated site patterns with integer weights; FP64.
+
+One step = one full evaluation: upload of the branch lengths, P/dP build,
+every post-order and pre-order level, reductions and download of
+(logL, grad[198]).  Inputs (tip codes, model) are resident in HBM before the
+timed region.  value = patterns * steps / wall seconds.
+
+Prints one JSON line on stdout; progress goes to stderr.
+"""
+
+import argparse
+import importlib.util
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def load_package():
+    pkg = os.path.join(ROOT, "mcphylo.jl_amd")
+    spec = importlib.util.spec_from_file_location(
+        "mcphylo_amd", os.path.join(pkg, "__init__.py"), submodule_search_locations=[pkg])
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules["mcphylo_amd"] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+WORKLOADS = {
+    "gtr100": ("100 taxa random topology, t~Exp(0.05), 4-state GTR, 4 gamma cats (alpha 0.5), "
+               "10,000 weighted patterns, FP64 logL + d logL/d t"),
+}
+
+
+def cpu_baseline(mcp, w, blv, budget_s):
+    """The CPU oracle (reference-algorithm restatement) on this host's cores,
+    timed on the same workload for about budget_s seconds."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    threads = min(os.cpu_count() or 1, 16)
+    oracle.set_threads(threads)
+    oracle.logl_grad(w.ta, w.x, w.U, w.D, w.Uinv, w.mu, w.rates, w.pi, blv, weights=w.weights)
+    n, t0 = 0, time.perf_counter()
+    while True:
+        oracle.logl_grad(w.ta, w.x, w.U, w.D, w.Uinv, w.mu, w.rates, w.pi, blv, weights=w.weights)
+        n += 1
+        dt = time.perf_counter() - t0
+        if dt >= budget_s and n >= 2:
+            break
+    return {"value": w.C * n / dt, "unit": "site-pattern evals/s", "cores": threads,
+            "kind": "port",
+            "sample": f"{n} full logL+grad evaluations of the same workload "
+                      f"({w.C} patterns, {w.N} nodes) with the C oracle, {threads} OpenMP threads"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="gtr100", choices=sorted(WORKLOADS))
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
+              "(the path shards over site patterns across GPUs by design, not built here)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    mcp = load_package()
+    if mcp._lib.device_count() < 1:
+        log("no HIP device visible")
+        sys.exit(1)
+    log(f"building workload {args.workload}")
+    w = mcp.synthetic.make_workload(args.workload)
+    blv = mcp.get_branchlength_vector(w.tree)
+    plan = mcp.Plan(w.ta.parent, w.x, len(w.rates), weights=w.weights)
+    plan.set_model(mcp.Model(w.U, w.D, w.Uinv, w.mu, w.rates, w.pi))
+    for _ in range(args.warmup):
+        plan.logl_grad(blv)
+    log(f"warmup done ({args.warmup}); timing {args.steps} steps")
+
+    # timed region: eager launches, each kernel bracketed by HIP events
+    plan.set_timing(True)
+    phase_ms = np.zeros(5)
+    launch_ms = None
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        plan.logl_grad(blv)
+        ms, ph = plan.launch_times()
+        launch_ms = ms if launch_ms is None else launch_ms + ms
+        phase_ms += plan.timing()
+    wall = time.perf_counter() - t0
+    phases = launch_ms.copy()
+    _, ph = plan.launch_times()
+    plan.set_timing(False)
+    value = w.C * args.steps / wall
+    log(f"eager: {1e3 * wall / args.steps:.3f} ms/step")
+
+    # graph replay of the same evaluation
+    plan.use_graph(True)
+    for _ in range(3):
+        plan.logl_grad(blv)
+    t1 = time.perf_counter()
+    for _ in range(args.steps):
+        plan.logl_grad(blv)
+    gwall = time.perf_counter() - t1
+    plan.use_graph(False)
+    gvalue = w.C * args.steps / gwall
+    log(f"graph: {1e3 * gwall / args.steps:.3f} ms/step")
+
+    bytes_fwd, bytes_adj = plan.algorithmic_bytes()
+    fwd_ms = float(np.sum(phases[ph == 2])) / args.steps
+    adj_ms = float(np.sum(phases[ph == 3])) / args.steps
+    n_fwd = int(np.sum(ph == 2))
+    n_adj = int(np.sum(ph == 3))
+    if adj_ms >= fwd_ms:
+        kname, kbytes, kms, nl = "k_adj_level<4>", bytes_adj, adj_ms, n_adj
+    else:
+        kname, kbytes, kms, nl = "k_fwd_level<4>", bytes_fwd, fwd_ms, n_fwd
+    achieved = kbytes / (kms * 1e-3) / 1e9
+    result = {
+        "metric": "logL+grad site-pattern evals/sec",
+        "value": value,
+        "unit": "site-pattern evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * wall / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"{args.workload}: {WORKLOADS[args.workload]}",
+                   "taxa": int(len(w.ta.leaf_nums)), "nodes": int(w.N), "states": int(w.S),
+                   "patterns": int(w.C), "rate_categories": int(len(w.rates)),
+                   "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": kname, "launches_per_step": nl,
+                     "algorithmic_bytes_per_step": kbytes,
+                     "kernel_ms_per_step": kms},
+        "graph_value": gvalue,
+        "phase_ms_per_step": {"P_build": float(np.sum(phases[ph == 1])) / args.steps,
+                              "post_order": fwd_ms, "pre_order": adj_ms,
+                              "reduce": float(np.sum(phases[ph == 4])) / args.steps},
+        "algorithmic_bytes_per_step": {"post_order": bytes_fwd, "pre_order": bytes_adj},
+    }
+    if not args.no_cpu:
+        log("timing CPU baseline")
+        result["cpu_baseline"] = cpu_baseline(mcp, w, blv, args.cpu_seconds)
+    else:
+        result["cpu_baseline"] = None
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,171 @@
+/*
+ * mcphylo_gpu.h -- C ABI of the MI355X (gfx950) pruning likelihood engine.
+ *
+ * Drop-in boundary for MCPhylo.jl's PhyloDist hot path: the engine replaces
+ * the body of
+ *   logpdf(d::PhyloDist, x)         src/distributions/Phylodist.jl:107-122
+ *   gradlogpdf(d::PhyloDist, x)     src/distributions/Phylodist.jl:124-138
+ *   logpdf(d::MultiplePhyloDist, x) src/distributions/Phylodist.jl:281-288
+ *   __logpdf(d::MultiplePhyloDist)  src/distributions/Phylodist.jl:290-297
+ * i.e. FelsensteinFunction (src/Likelihood/LikelihoodCalculator_Node.jl:3-114)
+ * and the loop nests of src/Likelihood/VectorizedFunctions.jl:2-213.
+ *
+ * Conventions (those of the reference):
+ *  - nodes are numbered 1..N, the root is N; a parent array gives the parent
+ *    num of node num i+1 (0 for the root);
+ *  - branch lengths and gradients are vectors of length N-1 indexed by num;
+ *  - tip partials use the reference layout x[s, c, v] (column-major: state
+ *    fastest, then site/pattern, then node);
+ *  - U, Uinv are S x S column-major; Q = U diag(D) Uinv; P(t) =
+ *    U diag(exp(mu * t * D * rate)) Uinv (VectorizedFunctions.jl:116-168);
+ *  - with R > 1 categories the log-likelihood is the sum over every
+ *    (pattern, category) pair, as in VectorizedFunctions.jl:37-40,76-87;
+ *  - numeric failure (log 0, NaN inputs) is returned as -Inf/NaN values,
+ *    never as an error status (reference: simulation.jl:246,263-267).
+ * All pointers are host memory owned by the caller; nothing is retained
+ * after a call returns.  A handle is used by one host thread at a time;
+ * distinct handles may be used concurrently (one HIP stream each).
+ */
+#ifndef MCPHYLO_GPU_H
+#define MCPHYLO_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct mcp_engine mcp_plan;  /* one tree: a batch of one item */
+typedef struct mcp_engine mcp_batch; /* many (tree, chain) items */
+
+/* Status codes; the Julia wrapper maps them to ArgumentError,
+ * DimensionMismatch and ErrorException. */
+enum {
+    MCP_OK = 0,
+    MCP_ERR_INVALID = 1,     /* bad argument (null pointer, malformed tree) */
+    MCP_ERR_DIMENSION = 2,   /* sizes that do not agree (DimensionMismatch) */
+    MCP_ERR_COMPLEX = 3,     /* non-finite model input */
+    MCP_ERR_NOMEM = 4,       /* device or host allocation failed */
+    MCP_ERR_HIP = 5,         /* HIP runtime error */
+    MCP_ERR_UNSUPPORTED = 6, /* state count or tip alphabet outside the built kernels */
+    MCP_ERR_STATE = 7        /* model not set before evaluation */
+};
+
+/* Merge identical alignment columns into weighted patterns (the reference
+ * has no compression; sums are unchanged up to summation order). */
+#define MCP_FLAG_COMPRESS_PATTERNS 1u
+
+/* ---- single tree (PhyloDist) -------------------------------------------- */
+
+typedef struct mcp_plan_desc {
+    int32_t n_nodes;                /* N (root = N) */
+    int32_t n_states;               /* S */
+    int32_t n_patterns;             /* C (sites, or patterns with weights) */
+    int32_t n_rates;                /* R rate categories */
+    const int32_t* parent;          /* [N] */
+    const double* tip_partials;     /* [S*C*N] x[s,c,v]; only leaf slices are read */
+    const double* pattern_weights;  /* [C] or NULL (all ones) */
+    int32_t device;                 /* HIP device ordinal */
+    uint32_t flags;                 /* MCP_FLAG_* */
+} mcp_plan_desc;
+
+/* Replaces the per-call setup of Phylodist.jl:108-120 (post_order, leaf
+ * lookup, my_repeat, deepcopy): uploads the tips once as state codes. */
+int mcp_plan_create(mcp_plan** out, const mcp_plan_desc* desc);
+void mcp_plan_destroy(mcp_plan* plan);
+
+/* The eigensystem d.substitution_model(base_freq, rates) returns
+ * (Phylodist.jl:110-113), the category rates d.rates and base_freq.
+ * U, Uinv: [S*S] column-major; D, pi: [S]; rates: [R]. */
+int mcp_set_model(mcp_plan* plan, const double* U, const double* D, const double* Uinv,
+                  double mu, const double* rates, const double* pi);
+
+/* New topology over the same leaves (after an NNI move, refraction.jl:65). */
+int mcp_set_topology(mcp_plan* plan, const int32_t* parent);
+
+/* logpdf(d::PhyloDist, x) (Phylodist.jl:107-122): blv [N-1] by num. */
+int mcp_logl(mcp_plan* plan, const double* blv, double* ll);
+
+/* gradlogpdf(d::PhyloDist, x) (Phylodist.jl:124-138): ll and grad [N-1]. */
+int mcp_logl_grad(mcp_plan* plan, const double* blv, double* ll, double* grad);
+
+/* ---- ragged batch of (tree, chain) items (MultiplePhyloDist and beyond) -- */
+
+typedef struct mcp_dataset_desc {
+    int32_t n_states;
+    int32_t n_patterns;
+    int32_t n_leaves;
+    const int32_t* leaf_nums;       /* [n_leaves] node nums of the leaves */
+    const double* tip_partials;     /* [S*C*n_leaves] x[s,c,j] for leaf leaf_nums[j] */
+    const double* pattern_weights;  /* [C] or NULL */
+    uint32_t flags;                 /* MCP_FLAG_* */
+} mcp_dataset_desc;
+
+typedef struct mcp_item_desc {
+    int32_t dataset;                /* index into mcp_batch_desc.datasets */
+    int32_t n_nodes;
+    int32_t n_rates;
+    const int32_t* parent;          /* [n_nodes]; leaves must be the dataset's leaf_nums */
+} mcp_item_desc;
+
+typedef struct mcp_batch_desc {
+    int32_t n_datasets;
+    const mcp_dataset_desc* datasets;
+    int32_t n_items;
+    const mcp_item_desc* items;
+    int32_t device;
+    int32_t tile;                   /* patterns per task: 64, 128 or 256; 0 = default */
+} mcp_batch_desc;
+
+/* MultiplePhyloDist (Phylodist.jl:143-272) generalised to uneven trees,
+ * state counts, pattern counts and per-chain branch lengths. */
+int mcp_batch_create(mcp_batch** out, const mcp_batch_desc* desc);
+void mcp_batch_destroy(mcp_batch* batch);
+int mcp_batch_set_model(mcp_batch* batch, int32_t item, const double* U, const double* D,
+                        const double* Uinv, double mu, const double* rates, const double* pi);
+int mcp_batch_set_topology(mcp_batch* batch, int32_t item, const int32_t* parent);
+
+/* blv_concat: the items' branch-length vectors back to back (N_i - 1 each,
+ * item order); ll: [n_items]; grad_concat: same layout as blv_concat.
+ * logpdf(::MultiplePhyloDist) is the sum of ll; __logpdf is (ll, grad). */
+int mcp_batch_logl(mcp_batch* batch, const double* blv_concat, double* ll);
+int mcp_batch_logl_grad(mcp_batch* batch, const double* blv_concat, double* ll,
+                        double* grad_concat);
+
+/* ---- launch control ----------------------------------------------------- */
+
+/* Capture the whole evaluation (upload of branch lengths, P/dP build, every
+ * level, reductions, download) into HIP graphs that later calls replay; a
+ * topology change re-captures.  on = 0 switches back to eager launches.
+ * Applies to plans and batches (mcp_plan and mcp_batch are one type). */
+int mcp_use_graph(mcp_batch* batch, int32_t on);
+
+/* Time later evaluations with HIP events on the engine's stream (on != 0).
+ * ms[0] whole sequence (first upload to last download); ms[1..4] summed
+ * kernel durations of the P/dP build, post-order levels, pre-order levels
+ * and reductions -- of the last evaluation. */
+int mcp_set_timing(mcp_batch* batch, int32_t on);
+int mcp_get_timing(mcp_batch* batch, double* ms5);
+
+/* Durations (ms) and phases (1..4 as above) of every kernel launch of the
+ * last timed evaluation, in launch order; returns the number of launches
+ * (or a negative status).  At most cap entries are written. */
+int mcp_get_launch_times(mcp_batch* batch, double* ms, int32_t* phase, int32_t cap);
+
+/* Algorithmic HBM bytes of one evaluation under this engine's layout:
+ * bytes[0] post-order levels, bytes[1] pre-order levels (compulsory slab
+ * reads and writes counted once each; tip codes one byte per pattern). */
+int mcp_get_bytes(mcp_batch* batch, double* bytes2);
+
+/* Number of items, and total patterns over items (the unit of the
+ * site-pattern evals/s metric). */
+int mcp_get_sizes(mcp_batch* batch, int64_t* sizes2);
+
+/* Message of the last failed call on this host thread ("" if none). */
+const char* mcp_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MCPHYLO_GPU_H */

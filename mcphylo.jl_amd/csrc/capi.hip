@@ -1,0 +1,1044 @@
+// Host runtime and C ABI of the pruning engine.
+//
+// The engine owns, per batch: the datasets (tip state codes + code table +
+// pattern weights), the items (topology tables, level schedules, models),
+// one HIP stream, all device memory, pinned staging buffers and optional
+// HIP graphs of the evaluation.  Everything topology-derived is rebuilt on
+// the host (plan builder below) and uploaded; evaluation then is
+//   H2D branch lengths -> k_build_P -> k_tip_tables -> post-order levels ->
+//   [pre-order levels] -> k_reduce -> D2H (ll, grad)
+// on the engine's stream.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <new>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "engine.h"
+#include "mcphylo_gpu.h"
+
+namespace mcp {
+hipError_t launch_build_P(const ModelDev*, const int32_t*, int, long long, hipStream_t);
+hipError_t launch_tip_tables(const ModelDev*, const int32_t*, int, long long, hipStream_t);
+hipError_t launch_level(bool, int, const Item*, const Job*, const int32_t*, int, double*, int, int,
+                        hipStream_t);
+hipError_t launch_reduce(const RedOp*, int, const double*, double*, hipStream_t);
+bool supported_states(int S);
+}  // namespace mcp
+
+using namespace mcp;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+struct Status {
+    int code;
+    std::string msg;
+};
+
+int fail(int code, const std::string& msg) {
+    g_last_error = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                       \
+    do {                                                                                    \
+        hipError_t e_ = (expr);                                                             \
+        if (e_ != hipSuccess) {                                                             \
+            return fail(e_ == hipErrorOutOfMemory ? MCP_ERR_NOMEM : MCP_ERR_HIP,            \
+                        std::string(#expr) + ": " + hipGetErrorString(e_));                 \
+        }                                                                                   \
+    } while (0)
+
+size_t round_up(size_t x, size_t m) { return (x + m - 1) / m * m; }
+
+// Device buffer with capacity (bytes); grows by reallocation.
+struct DevBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+    ~DevBuf() {
+        if (p) (void)hipFree(p);
+    }
+    int ensure(size_t bytes) {
+        if (bytes <= cap && p) return MCP_OK;
+        if (p) {
+            (void)hipFree(p);
+            p = nullptr;
+            cap = 0;
+        }
+        size_t b = std::max<size_t>(bytes, 256);
+        hipError_t e = hipMalloc(&p, b);
+        if (e != hipSuccess) {
+            p = nullptr;
+            return fail(MCP_ERR_NOMEM, std::string("hipMalloc of ") + std::to_string(b) +
+                                           " bytes failed: " + hipGetErrorString(e));
+        }
+        cap = b;
+        return MCP_OK;
+    }
+    template <class T>
+    T* as() const { return static_cast<T*>(p); }
+};
+
+struct PinnedBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+    ~PinnedBuf() {
+        if (p) (void)hipHostFree(p);
+    }
+    int ensure(size_t bytes) {
+        if (bytes <= cap && p) return MCP_OK;
+        if (p) {
+            (void)hipHostFree(p);
+            p = nullptr;
+            cap = 0;
+        }
+        size_t b = std::max<size_t>(bytes, 256);
+        hipError_t e = hipHostMalloc(&p, b, hipHostMallocDefault);
+        if (e != hipSuccess) {
+            p = nullptr;
+            return fail(MCP_ERR_NOMEM, "hipHostMalloc failed");
+        }
+        cap = b;
+        return MCP_OK;
+    }
+    template <class T>
+    T* as() const { return static_cast<T*>(p); }
+};
+
+struct DatasetHost {
+    int S = 0, C = 0, Cp = 0, K = 0, nleaves = 0;
+    std::vector<int32_t> leaf_nums;          // row j of codes is leaf leaf_nums[j]
+    std::unordered_map<int32_t, int32_t> row_of;
+    std::vector<uint8_t> codes;              // [nleaves][Cp]
+    std::vector<double> tiptab;              // [K][S]
+    std::vector<double> weights;             // [Cp]
+    size_t codes_off = 0, tiptab_off = 0, weights_off = 0;  // offsets in device arenas
+};
+
+struct ItemHost {
+    int dataset = 0, N = 0, R = 0, S = 0, nint = 0;
+    std::vector<int32_t> parent;             // [N]
+    std::vector<int32_t> child_ptr, children;  // CSR by node index (num-1), children ascending
+    std::vector<NodeInfo> nodes;
+    std::vector<int32_t> height, depth;
+    int max_height = 0, max_depth = 0;
+    int g_slots[2] = {0, 0};
+    // model
+    bool model_set = false;
+    std::vector<double> U, D, Uinv, rates, pi;
+    double mu = 0.0;
+    // layout
+    size_t F_off = 0;      // doubles into F arena
+    size_t G_off[2] = {0, 0};  // doubles into G arenas
+    size_t P_off = 0;      // doubles into P/dP arenas ((N-1)*R*S*S)
+    size_t PT_off = 0;     // doubles into PT/dPT arenas (nleaves*R*K*S)
+    size_t model_off = 0;  // doubles into model arena
+    size_t blv_off = 0;    // doubles into blv (N-1)
+    int parts_off = 0;
+    int out_off = 0;
+    int ntiles = 0;
+    int children_off = 0;  // int32 into children arena
+    int nodes_off = 0;     // NodeInfo into nodes arena
+    int leafof_off = 0;    // int32 into leaf_of arena
+};
+
+struct Launch {
+    bool adjoint;
+    int S;
+    int job_begin, njobs;
+    int taskoff_begin;  // index into task_off arena (njobs+1 entries)
+    int ntasks;
+};
+
+}  // namespace
+
+struct mcp_engine {
+    int device = 0;
+    int tile = 128;
+    int num_cu = 256;
+    hipStream_t stream = nullptr;
+    std::vector<DatasetHost> ds;
+    std::vector<ItemHost> items;
+    // device arenas
+    DevBuf dF, dG0, dG1, dP, ddP, dPT, ddPT, dparts, dout, dblv, dmodel, dcodes, dweights, dtiptab,
+        dnodes, dchildren, dleafof, ditems, dmodels, djobs, dtaskoff, dred, delemP, delemT;
+    PinnedBuf hblv, hout;
+    // schedule
+    std::vector<Launch> fwd, adj;
+    std::vector<Job> jobs;
+    std::vector<int32_t> taskoff;
+    std::vector<RedOp> red;  // ll ops first (n_items), then gradient ops
+    int n_ll_ops = 0;
+    long long nelemP = 0, nelemT = 0;
+    size_t total_blv = 0, total_out = 0;
+    long long total_patterns = 0;
+    double bytes_fwd = 0.0, bytes_adj = 0.0;
+    // graphs
+    bool use_graph = false;
+    hipGraphExec_t graph_exec[2] = {nullptr, nullptr};  // [0] logl, [1] logl+grad
+    // timing: one event pair around every kernel launch of an evaluation
+    bool timing = false;
+    hipEvent_t ev_begin = nullptr, ev_end = nullptr;
+    std::vector<hipEvent_t> kev;  // [2k] before launch k, [2k+1] after
+    std::vector<int> kphase;      // phase of launch k: 1 P build, 2 post-order, 3 pre-order, 4 reduce
+    std::vector<double> kms;      // duration of launch k in the last timed evaluation
+    int nk = 0;
+    double last_ms[5] = {0, 0, 0, 0, 0};
+
+    ~mcp_engine() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        drop_graphs();
+        for (auto& e : kev)
+            if (e) (void)hipEventDestroy(e);
+        if (ev_begin) (void)hipEventDestroy(ev_begin);
+        if (ev_end) (void)hipEventDestroy(ev_end);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+    void drop_graphs() {
+        for (auto& g : graph_exec) {
+            if (g) (void)hipGraphExecDestroy(g);
+            g = nullptr;
+        }
+    }
+};
+
+namespace {
+
+struct DeviceGuard {
+    int prev = -1;
+    explicit DeviceGuard(int dev) {
+        if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+        if (prev != dev) (void)hipSetDevice(dev);
+    }
+    ~DeviceGuard() {
+        if (prev >= 0) (void)hipSetDevice(prev);
+    }
+};
+
+// ---------------------------------------------------------------------------
+// datasets: tip partials -> state codes (+ optional pattern compression)
+// ---------------------------------------------------------------------------
+
+int build_dataset(const mcp_dataset_desc& d, int tile, DatasetHost& out) {
+    if (d.n_states < 1 || d.n_patterns < 1 || d.n_leaves < 1 || !d.leaf_nums || !d.tip_partials)
+        return fail(MCP_ERR_INVALID, "dataset: sizes must be positive and pointers non-null");
+    if (!supported_states(d.n_states))
+        return fail(MCP_ERR_UNSUPPORTED, "dataset: no kernel built for " +
+                                             std::to_string(d.n_states) + " states");
+    const int S = d.n_states, C = d.n_patterns, L = d.n_leaves;
+    out.S = S;
+    out.nleaves = L;
+    out.leaf_nums.assign(d.leaf_nums, d.leaf_nums + L);
+    // sort leaves by num so rows are canonical
+    std::vector<int> order(L);
+    for (int j = 0; j < L; ++j) order[j] = j;
+    std::sort(order.begin(), order.end(),
+              [&](int a, int b) { return d.leaf_nums[a] < d.leaf_nums[b]; });
+    std::vector<int32_t> sorted_nums(L);
+    for (int j = 0; j < L; ++j) sorted_nums[j] = d.leaf_nums[order[j]];
+    for (int j = 1; j < L; ++j)
+        if (sorted_nums[j] == sorted_nums[j - 1])
+            return fail(MCP_ERR_INVALID, "dataset: duplicate leaf num");
+    out.leaf_nums = sorted_nums;
+    out.row_of.clear();
+    for (int j = 0; j < L; ++j) out.row_of[sorted_nums[j]] = j;
+
+    // code table over distinct tip vectors
+    std::unordered_map<std::string, int> code_of;
+    std::vector<uint8_t> codes((size_t)L * C);
+    out.tiptab.clear();
+    for (int j = 0; j < L; ++j) {
+        const double* xl = d.tip_partials + (size_t)S * C * order[j];
+        for (int c = 0; c < C; ++c) {
+            std::string key(reinterpret_cast<const char*>(xl + (size_t)S * c), sizeof(double) * S);
+            auto it = code_of.find(key);
+            int code;
+            if (it == code_of.end()) {
+                code = (int)code_of.size();
+                if (code > 255)
+                    return fail(MCP_ERR_UNSUPPORTED,
+                                "dataset: more than 256 distinct tip vectors");
+                code_of.emplace(key, code);
+                out.tiptab.insert(out.tiptab.end(), xl + (size_t)S * c, xl + (size_t)S * (c + 1));
+            } else {
+                code = it->second;
+            }
+            codes[(size_t)j * C + c] = (uint8_t)code;
+        }
+    }
+    out.K = (int)code_of.size();
+
+    std::vector<double> w(C, 1.0);
+    if (d.pattern_weights) w.assign(d.pattern_weights, d.pattern_weights + C);
+    int Cu = C;
+    std::vector<int> colmap(C);
+    for (int c = 0; c < C; ++c) colmap[c] = c;
+    if (d.flags & MCP_FLAG_COMPRESS_PATTERNS) {
+        std::unordered_map<std::string, int> col_of;
+        std::vector<double> wu;
+        std::vector<int> keep;
+        std::string key(L, '\0');
+        for (int c = 0; c < C; ++c) {
+            for (int j = 0; j < L; ++j) key[j] = (char)codes[(size_t)j * C + c];
+            auto it = col_of.find(key);
+            if (it == col_of.end()) {
+                col_of.emplace(key, (int)keep.size());
+                keep.push_back(c);
+                wu.push_back(w[c]);
+            } else {
+                wu[it->second] += w[c];
+            }
+        }
+        Cu = (int)keep.size();
+        colmap = keep;
+        w = wu;
+    }
+    out.C = Cu;
+    out.Cp = (int)round_up((size_t)Cu, (size_t)tile);
+    out.codes.assign((size_t)L * out.Cp, 0);
+    for (int j = 0; j < L; ++j)
+        for (int c = 0; c < Cu; ++c)
+            out.codes[(size_t)j * out.Cp + c] = codes[(size_t)j * C + colmap[c]];
+    out.weights.assign(out.Cp, 0.0);
+    for (int c = 0; c < Cu; ++c) out.weights[c] = w[c];
+    return MCP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// items: topology tables
+// ---------------------------------------------------------------------------
+
+int build_topology(ItemHost& it, const DatasetHost& d, const int32_t* parent) {
+    const int N = it.N;
+    if (N < 2) return fail(MCP_ERR_INVALID, "tree needs at least two nodes");
+    it.parent.assign(parent, parent + N);
+    if (it.parent[N - 1] != 0) return fail(MCP_ERR_INVALID, "the root must be node num N");
+    std::vector<int32_t> cnt(N, 0);
+    for (int v = 0; v < N - 1; ++v) {
+        int p = it.parent[v];
+        if (p < 1 || p > N || p == v + 1)
+            return fail(MCP_ERR_INVALID, "parent num out of range at node " + std::to_string(v + 1));
+        cnt[p - 1]++;
+    }
+    it.child_ptr.assign(N + 1, 0);
+    for (int v = 0; v < N; ++v) it.child_ptr[v + 1] = it.child_ptr[v] + cnt[v];
+    it.children.assign(N - 1, 0);
+    std::vector<int32_t> fill(it.child_ptr.begin(), it.child_ptr.end() - 1);
+    for (int v = 0; v < N - 1; ++v) it.children[fill[it.parent[v] - 1]++] = v;
+    // depth by walking up (detects cycles)
+    it.depth.assign(N, -1);
+    it.depth[N - 1] = 0;
+    for (int v = 0; v < N; ++v) {
+        std::vector<int> path;
+        int x = v, steps = 0;
+        while (it.depth[x] < 0) {
+            path.push_back(x);
+            x = it.parent[x] - 1;
+            if (++steps > N) return fail(MCP_ERR_INVALID, "parent array has a cycle");
+        }
+        int dd = it.depth[x];
+        for (int k = (int)path.size() - 1; k >= 0; --k) it.depth[path[k]] = ++dd;
+    }
+    // leaves must match the dataset
+    int nleaves = 0;
+    for (int v = 0; v < N; ++v)
+        if (cnt[v] == 0) {
+            ++nleaves;
+            if (!d.row_of.count(v + 1))
+                return fail(MCP_ERR_DIMENSION,
+                            "leaf num " + std::to_string(v + 1) + " has no tip data");
+        }
+    if (nleaves != d.nleaves)
+        return fail(MCP_ERR_DIMENSION, "tree leaves and tip data disagree");
+    if (cnt[N - 1] == 0) return fail(MCP_ERR_INVALID, "root has no children");
+    // heights (post-order levels): process nodes by decreasing depth
+    std::vector<int> by_depth(N);
+    for (int v = 0; v < N; ++v) by_depth[v] = v;
+    std::stable_sort(by_depth.begin(), by_depth.end(),
+                     [&](int a, int b) { return it.depth[a] > it.depth[b]; });
+    it.height.assign(N, 0);
+    for (int v : by_depth) {
+        int h = 0;
+        for (int q = it.child_ptr[v]; q < it.child_ptr[v + 1]; ++q)
+            h = std::max(h, it.height[it.children[q]] + 1);
+        it.height[v] = h;
+    }
+    it.max_height = it.height[N - 1];
+    it.max_depth = *std::max_element(it.depth.begin(), it.depth.end());
+    // node records
+    it.nodes.assign(N, NodeInfo{});
+    it.nint = 0;
+    int fslot = 0, lidx = 0;
+    std::vector<int> gcount(it.max_depth + 2, 0);
+    for (int v = 0; v < N; ++v) {
+        NodeInfo& n = it.nodes[v];
+        n.first_child = it.child_ptr[v];
+        n.nchild = cnt[v];
+        n.is_root = (v == N - 1);
+        n.fslot = -1;
+        n.leaf = -1;
+        n.gslot = -1;
+        n.gbuf = it.depth[v] & 1;
+        n.lpart = -1;
+        n.gpart = -1;
+        if (cnt[v] == 0) {
+            n.leaf = d.row_of.at(v + 1);
+        } else {
+            it.nint++;
+            if (!n.is_root) {
+                n.fslot = fslot++;
+                n.gslot = gcount[it.depth[v]]++;
+            }
+        }
+    }
+    it.g_slots[0] = it.g_slots[1] = 0;
+    for (int dd = 0; dd <= it.max_depth; ++dd)
+        it.g_slots[dd & 1] = std::max(it.g_slots[dd & 1], gcount[dd]);
+    // partial-sum offsets (relative to the item; made absolute in layout)
+    const int per = it.R * it.ntiles;
+    for (int v = 0; v < N; ++v) {
+        NodeInfo& n = it.nodes[v];
+        if (n.nchild > 0) n.lpart = (lidx++) * per;
+    }
+    for (int v = 0; v < N - 1; ++v) it.nodes[v].gpart = (it.nint + v) * per;
+    (void)fslot;
+    return MCP_OK;
+}
+
+int item_from_desc(const mcp_item_desc& d, const std::vector<DatasetHost>& ds, int tile,
+                   ItemHost& it) {
+    if (d.dataset < 0 || d.dataset >= (int)ds.size())
+        return fail(MCP_ERR_INVALID, "item: dataset index out of range");
+    if (!d.parent || d.n_nodes < 2 || d.n_rates < 1)
+        return fail(MCP_ERR_INVALID, "item: bad sizes or null parent array");
+    const DatasetHost& D = ds[d.dataset];
+    it.dataset = d.dataset;
+    it.N = d.n_nodes;
+    it.R = d.n_rates;
+    it.S = D.S;
+    it.ntiles = D.Cp / tile;
+    return build_topology(it, D, d.parent);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// layout, schedule, upload
+// ---------------------------------------------------------------------------
+
+static int upload(mcp_engine* e, DevBuf& b, const void* src, size_t bytes) {
+    int st = b.ensure(bytes);
+    if (st) return st;
+    if (bytes) HIP_TRY(hipMemcpy(b.p, src, bytes, hipMemcpyHostToDevice));
+    (void)e;
+    return MCP_OK;
+}
+
+// Device descriptors and launch schedule; called after create and after any
+// topology change.  F/P/model arenas keep their layout (sizes do not depend
+// on topology); the pre-order ring buffers may grow.
+static int rebuild_schedule(mcp_engine* e) {
+    e->drop_graphs();
+    const int nitems = (int)e->items.size();
+    // pre-order ring buffers
+    size_t g0 = 0, g1 = 0;
+    for (auto& it : e->items) {
+        const DatasetHost& d = e->ds[it.dataset];
+        const size_t slab = (size_t)it.R * d.Cp * it.S;
+        it.G_off[0] = g0;
+        it.G_off[1] = g1;
+        g0 += (size_t)it.g_slots[0] * slab;
+        g1 += (size_t)it.g_slots[1] * slab;
+    }
+    int st;
+    if ((st = e->dG0.ensure(std::max<size_t>(g0, 1) * sizeof(double)))) return st;
+    if ((st = e->dG1.ensure(std::max<size_t>(g1, 1) * sizeof(double)))) return st;
+
+    // node/children/leaf_of tables
+    std::vector<NodeInfo> nodes;
+    std::vector<int32_t> children, leaf_of;
+    std::vector<Item> ditems(nitems);
+    std::vector<ModelDev> dmodels(nitems);
+    std::vector<int32_t> elemP(nitems + 1, 0), elemT(nitems + 1, 0);
+    e->bytes_fwd = e->bytes_adj = 0.0;
+    for (int i = 0; i < nitems; ++i) {
+        ItemHost& it = e->items[i];
+        const DatasetHost& d = e->ds[it.dataset];
+        it.nodes_off = (int)nodes.size();
+        it.children_off = (int)children.size();
+        it.leafof_off = (int)leaf_of.size();
+        for (const NodeInfo& n0 : it.nodes) {
+            NodeInfo n = n0;
+            if (n.lpart >= 0) n.lpart += it.parts_off;
+            if (n.gpart >= 0) n.gpart += it.parts_off;
+            nodes.push_back(n);
+            leaf_of.push_back(n0.leaf);
+        }
+        children.insert(children.end(), it.children.begin(), it.children.end());
+        Item& x = ditems[i];
+        x.S = it.S;
+        x.C = d.C;
+        x.Cp = d.Cp;
+        x.R = it.R;
+        x.K = d.K;
+        x.N = it.N;
+        x.ntiles = it.ntiles;
+        x.tile = e->tile;
+        elemP[i + 1] = elemP[i] + (it.N - 1) * it.R * it.S * it.S;
+        elemT[i + 1] = elemT[i] + (it.N - 1) * it.R * d.K * it.S;
+        // algorithmic bytes
+        const double w = 8.0 * it.S * it.R * d.C;
+        for (int v = 0; v < it.N; ++v) {
+            const NodeInfo& n = it.nodes[v];
+            if (n.nchild == 0) {
+                e->bytes_adj += d.C;  // leaf code read in the pre-order pass
+                continue;
+            }
+            for (int q = it.child_ptr[v]; q < it.child_ptr[v + 1]; ++q) {
+                const int u = it.children[q];
+                e->bytes_fwd += it.nodes[u].nchild ? w : (double)d.C;
+            }
+            if (!n.is_root) {
+                e->bytes_fwd += w;        // write F_v
+                e->bytes_adj += w;        // read G_v as a parent
+                e->bytes_adj += 2.0 * w;  // read F_v, write G_v as a child
+            }
+        }
+    }
+    if ((st = upload(e, e->dnodes, nodes.data(), nodes.size() * sizeof(NodeInfo)))) return st;
+    if ((st = upload(e, e->dchildren, children.data(), children.size() * sizeof(int32_t)))) return st;
+    if ((st = upload(e, e->dleafof, leaf_of.data(), leaf_of.size() * sizeof(int32_t)))) return st;
+    for (int i = 0; i < nitems; ++i) {
+        ItemHost& it = e->items[i];
+        const DatasetHost& d = e->ds[it.dataset];
+        Item& x = ditems[i];
+        x.nodes = e->dnodes.as<NodeInfo>() + it.nodes_off;
+        x.children = e->dchildren.as<int32_t>() + it.children_off;
+        x.codes = e->dcodes.as<uint8_t>() + d.codes_off;
+        x.weights = e->dweights.as<double>() + d.weights_off;
+        x.pi = e->dmodel.as<double>() + it.model_off + 2 * it.S * it.S + it.S + it.R;
+        x.P = e->dP.as<double>() + it.P_off;
+        x.dP = e->ddP.as<double>() + it.P_off;
+        x.PT = e->dPT.as<double>() + it.PT_off;
+        x.dPT = e->ddPT.as<double>() + it.PT_off;
+        x.F = e->dF.as<double>() + it.F_off;
+        x.G0 = e->dG0.as<double>() + it.G_off[0];
+        x.G1 = e->dG1.as<double>() + it.G_off[1];
+        ModelDev& m = dmodels[i];
+        const double* mb = e->dmodel.as<double>() + it.model_off;
+        m.U = mb;
+        m.D = mb + it.S * it.S;
+        m.Uinv = mb + it.S * it.S + it.S;
+        m.rates = mb + 2 * it.S * it.S + it.S;
+        m.blv = e->dblv.as<double>() + it.blv_off;
+        m.tiptab = e->dtiptab.as<double>() + d.tiptab_off;
+        m.leaf_of = e->dleafof.as<int32_t>() + it.leafof_off;
+        m.mu = it.mu;
+        m.S = it.S;
+        m.R = it.R;
+        m.N = it.N;
+        m.K = d.K;
+        m.P = e->dP.as<double>() + it.P_off;
+        m.dP = e->ddP.as<double>() + it.P_off;
+        m.PT = e->dPT.as<double>() + it.PT_off;
+        m.dPT = e->ddPT.as<double>() + it.PT_off;
+    }
+    if ((st = upload(e, e->ditems, ditems.data(), ditems.size() * sizeof(Item)))) return st;
+    if ((st = upload(e, e->dmodels, dmodels.data(), dmodels.size() * sizeof(ModelDev)))) return st;
+    if ((st = upload(e, e->delemP, elemP.data(), elemP.size() * sizeof(int32_t)))) return st;
+    if ((st = upload(e, e->delemT, elemT.data(), elemT.size() * sizeof(int32_t)))) return st;
+    e->nelemP = elemP[nitems];
+    e->nelemT = elemT[nitems];
+
+    // level schedule: post-order by height, pre-order by depth, per state count
+    e->jobs.clear();
+    e->taskoff.clear();
+    e->fwd.clear();
+    e->adj.clear();
+    int maxh = 0, maxd = 0;
+    std::vector<int> svals;
+    for (auto& it : e->items) {
+        maxh = std::max(maxh, it.max_height);
+        maxd = std::max(maxd, it.max_depth);
+        if (std::find(svals.begin(), svals.end(), it.S) == svals.end()) svals.push_back(it.S);
+    }
+    std::sort(svals.begin(), svals.end());
+    auto add_launch = [&](bool adjoint, int S, int level) {
+        Launch L;
+        L.adjoint = adjoint;
+        L.S = S;
+        L.job_begin = (int)e->jobs.size();
+        L.taskoff_begin = (int)e->taskoff.size();
+        int tasks = 0;
+        e->taskoff.push_back(0);
+        for (int i = 0; i < nitems; ++i) {
+            const ItemHost& it = e->items[i];
+            if (it.S != S) continue;
+            for (int v = 0; v < it.N; ++v) {
+                if (it.nodes[v].nchild == 0) continue;
+                const int lv = adjoint ? it.depth[v] : it.height[v];
+                if (lv != level) continue;
+                e->jobs.push_back(Job{i, v});
+                tasks += it.R * it.ntiles;
+                e->taskoff.push_back(tasks);
+            }
+        }
+        L.njobs = (int)e->jobs.size() - L.job_begin;
+        L.ntasks = tasks;
+        if (L.njobs == 0) {
+            e->taskoff.pop_back();
+            return;
+        }
+        (adjoint ? e->adj : e->fwd).push_back(L);
+    };
+    for (int h = 1; h <= maxh; ++h)
+        for (int S : svals) add_launch(false, S, h);
+    for (int dd = 0; dd <= maxd; ++dd)
+        for (int S : svals) add_launch(true, S, dd);
+    if ((st = upload(e, e->djobs, e->jobs.data(), e->jobs.size() * sizeof(Job)))) return st;
+    if ((st = upload(e, e->dtaskoff, e->taskoff.data(), e->taskoff.size() * sizeof(int32_t))))
+        return st;
+
+    // reductions: ll of every item first, then every branch gradient
+    e->red.clear();
+    for (auto& it : e->items) {
+        RedOp op;
+        op.dst = it.out_off;
+        op.start = it.parts_off;
+        op.len = it.nint * it.R * it.ntiles;
+        op.pad_ = 0;
+        e->red.push_back(op);
+    }
+    e->n_ll_ops = (int)e->red.size();
+    for (auto& it : e->items)
+        for (int v = 0; v < it.N - 1; ++v) {
+            RedOp op;
+            op.dst = it.out_off + 1 + v;
+            op.start = it.parts_off + (it.nint + v) * it.R * it.ntiles;
+            op.len = it.R * it.ntiles;
+            op.pad_ = 0;
+            e->red.push_back(op);
+        }
+    if ((st = upload(e, e->dred, e->red.data(), e->red.size() * sizeof(RedOp)))) return st;
+    return MCP_OK;
+}
+
+static int engine_create(mcp_engine** out, const mcp_batch_desc* d) {
+    if (!out || !d) return fail(MCP_ERR_INVALID, "null argument");
+    *out = nullptr;
+    if (d->n_datasets < 1 || d->n_items < 1 || !d->datasets || !d->items)
+        return fail(MCP_ERR_INVALID, "batch needs at least one dataset and one item");
+    int tile = d->tile ? d->tile : 128;
+    if (tile != 64 && tile != 128 && tile != 256)
+        return fail(MCP_ERR_INVALID, "tile must be 64, 128 or 256");
+    int ndev = 0;
+    HIP_TRY(hipGetDeviceCount(&ndev));
+    if (d->device < 0 || d->device >= ndev)
+        return fail(MCP_ERR_INVALID, "device ordinal out of range");
+    DeviceGuard guard(d->device);
+    std::unique_ptr<mcp_engine> e(new (std::nothrow) mcp_engine());
+    if (!e) return fail(MCP_ERR_NOMEM, "host allocation failed");
+    e->device = d->device;
+    e->tile = tile;
+    hipDeviceProp_t prop;
+    HIP_TRY(hipGetDeviceProperties(&prop, d->device));
+    e->num_cu = prop.multiProcessorCount;
+    HIP_TRY(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
+    int st;
+    e->ds.resize(d->n_datasets);
+    for (int k = 0; k < d->n_datasets; ++k)
+        if ((st = build_dataset(d->datasets[k], tile, e->ds[k]))) return st;
+    e->items.resize(d->n_items);
+    for (int i = 0; i < d->n_items; ++i)
+        if ((st = item_from_desc(d->items[i], e->ds, tile, e->items[i]))) return st;
+
+    // fixed layout
+    size_t codes = 0, tiptab = 0, weights = 0;
+    for (auto& D : e->ds) {
+        D.codes_off = codes;
+        D.tiptab_off = tiptab;
+        D.weights_off = weights;
+        codes += D.codes.size();
+        tiptab += D.tiptab.size();
+        weights += D.weights.size();
+    }
+    size_t F = 0, P = 0, PT = 0, model = 0, blv = 0;
+    int parts = 0, outs = 0;
+    e->total_patterns = 0;
+    for (auto& it : e->items) {
+        const DatasetHost& D = e->ds[it.dataset];
+        it.F_off = F;
+        F += (size_t)(it.nint - 1) * it.R * D.Cp * it.S;
+        it.P_off = P;
+        P += (size_t)(it.N - 1) * it.R * it.S * it.S;
+        it.PT_off = PT;
+        PT += (size_t)D.nleaves * it.R * D.K * it.S;
+        it.model_off = model;
+        model += 2 * (size_t)it.S * it.S + 2 * (size_t)it.S + it.R;
+        it.blv_off = blv;
+        blv += it.N - 1;
+        const long long nparts = (long long)(it.nint + it.N - 1) * it.R * it.ntiles;
+        if ((long long)parts + nparts > 0x7fffffffLL)
+            return fail(MCP_ERR_UNSUPPORTED, "batch too large for 32-bit partial-sum offsets");
+        it.parts_off = parts;
+        parts += (int)nparts;
+        it.out_off = outs;
+        outs += it.N;
+        e->total_patterns += D.C;
+    }
+    e->total_blv = blv;
+    e->total_out = outs;
+    if ((st = e->dF.ensure(std::max<size_t>(F, 1) * sizeof(double)))) return st;
+    if ((st = e->dP.ensure(P * sizeof(double)))) return st;
+    if ((st = e->ddP.ensure(P * sizeof(double)))) return st;
+    if ((st = e->dPT.ensure(std::max<size_t>(PT, 1) * sizeof(double)))) return st;
+    if ((st = e->ddPT.ensure(std::max<size_t>(PT, 1) * sizeof(double)))) return st;
+    if ((st = e->dmodel.ensure(model * sizeof(double)))) return st;
+    if ((st = e->dblv.ensure(blv * sizeof(double)))) return st;
+    if ((st = e->dparts.ensure((size_t)parts * sizeof(double)))) return st;
+    if ((st = e->dout.ensure((size_t)outs * sizeof(double)))) return st;
+    if ((st = e->hblv.ensure(blv * sizeof(double)))) return st;
+    if ((st = e->hout.ensure((size_t)outs * sizeof(double)))) return st;
+    HIP_TRY(hipMemset(e->dparts.p, 0, (size_t)parts * sizeof(double)));
+    {
+        std::vector<uint8_t> hc(codes);
+        std::vector<double> ht(tiptab), hw(weights);
+        for (auto& D : e->ds) {
+            std::copy(D.codes.begin(), D.codes.end(), hc.begin() + D.codes_off);
+            std::copy(D.tiptab.begin(), D.tiptab.end(), ht.begin() + D.tiptab_off);
+            std::copy(D.weights.begin(), D.weights.end(), hw.begin() + D.weights_off);
+        }
+        if ((st = upload(e.get(), e->dcodes, hc.data(), hc.size()))) return st;
+        if ((st = upload(e.get(), e->dtiptab, ht.data(), ht.size() * sizeof(double)))) return st;
+        if ((st = upload(e.get(), e->dweights, hw.data(), hw.size() * sizeof(double)))) return st;
+    }
+    if ((st = rebuild_schedule(e.get()))) return st;
+    *out = e.release();
+    return MCP_OK;
+}
+
+static int engine_set_model(mcp_engine* e, int item, const double* U, const double* D,
+                            const double* Uinv, double mu, const double* rates, const double* pi) {
+    if (!e) return fail(MCP_ERR_INVALID, "null handle");
+    if (item < 0 || item >= (int)e->items.size()) return fail(MCP_ERR_INVALID, "item out of range");
+    if (!U || !D || !Uinv || !rates || !pi) return fail(MCP_ERR_INVALID, "null model array");
+    ItemHost& it = e->items[item];
+    const int S = it.S, R = it.R;
+    std::vector<double> buf;
+    buf.insert(buf.end(), U, U + S * S);
+    buf.insert(buf.end(), D, D + S);
+    buf.insert(buf.end(), Uinv, Uinv + S * S);
+    buf.insert(buf.end(), rates, rates + R);
+    buf.insert(buf.end(), pi, pi + S);
+    for (double x : buf)
+        if (!std::isfinite(x)) return fail(MCP_ERR_COMPLEX, "model arrays must be finite");
+    if (!std::isfinite(mu)) return fail(MCP_ERR_COMPLEX, "mu must be finite");
+    DeviceGuard guard(e->device);
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    HIP_TRY(hipMemcpy(e->dmodel.as<double>() + it.model_off, buf.data(), buf.size() * sizeof(double),
+                      hipMemcpyHostToDevice));
+    const bool mu_changed = !it.model_set || it.mu != mu;
+    it.U.assign(U, U + S * S);
+    it.D.assign(D, D + S);
+    it.Uinv.assign(Uinv, Uinv + S * S);
+    it.rates.assign(rates, rates + R);
+    it.pi.assign(pi, pi + S);
+    it.mu = mu;
+    it.model_set = true;
+    if (mu_changed) {
+        // mu lives in the device ModelDev record
+        ModelDev md;
+        HIP_TRY(hipMemcpy(&md, e->dmodels.as<ModelDev>() + item, sizeof(ModelDev),
+                          hipMemcpyDeviceToHost));
+        md.mu = mu;
+        HIP_TRY(hipMemcpy(e->dmodels.as<ModelDev>() + item, &md, sizeof(ModelDev),
+                          hipMemcpyHostToDevice));
+    }
+    return MCP_OK;
+}
+
+static int engine_set_topology(mcp_engine* e, int item, const int32_t* parent) {
+    if (!e || !parent) return fail(MCP_ERR_INVALID, "null argument");
+    if (item < 0 || item >= (int)e->items.size()) return fail(MCP_ERR_INVALID, "item out of range");
+    DeviceGuard guard(e->device);
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    ItemHost tmp = e->items[item];
+    int st = build_topology(tmp, e->ds[tmp.dataset], parent);
+    if (st) return st;
+    if (tmp.nint != e->items[item].nint)
+        return fail(MCP_ERR_DIMENSION, "new topology changes the number of internal nodes");
+    e->items[item] = std::move(tmp);
+    st = rebuild_schedule(e);
+    if (st) return st;
+    // the ModelDev records were rebuilt from host copies; mu included
+    return MCP_OK;
+}
+
+// Enqueue one evaluation on the engine's stream (no synchronisation).  In
+// timing mode (eager launches only) every kernel launch is bracketed by a
+// pair of HIP events on the same stream.
+static int enqueue_eval(mcp_engine* e, bool grad) {
+    hipStream_t st = e->stream;
+    const bool tm = e->timing && !e->use_graph;
+    e->nk = 0;
+    auto before = [&](int phase) -> int {
+        if (!tm) return MCP_OK;
+        const size_t k = (size_t)e->nk;
+        while (e->kev.size() < 2 * k + 2) {
+            hipEvent_t ev;
+            HIP_TRY(hipEventCreate(&ev));
+            e->kev.push_back(ev);
+        }
+        if (e->kphase.size() < k + 1) e->kphase.resize(k + 1);
+        e->kphase[k] = phase;
+        HIP_TRY(hipEventRecord(e->kev[2 * k], st));
+        return MCP_OK;
+    };
+    auto after = [&]() -> int {
+        if (!tm) return MCP_OK;
+        HIP_TRY(hipEventRecord(e->kev[2 * (size_t)e->nk + 1], st));
+        e->nk++;
+        return MCP_OK;
+    };
+    int rc;
+#define TIMED(phase, call)                 \
+    if ((rc = before(phase))) return rc;   \
+    HIP_TRY(call);                         \
+    if ((rc = after())) return rc;
+    if (tm) HIP_TRY(hipEventRecord(e->ev_begin, st));
+    HIP_TRY(hipMemcpyAsync(e->dblv.p, e->hblv.p, e->total_blv * sizeof(double),
+                           hipMemcpyHostToDevice, st));
+    TIMED(1, launch_build_P(e->dmodels.as<ModelDev>(), e->delemP.as<int32_t>(),
+                            (int)e->items.size(), e->nelemP, st));
+    TIMED(1, launch_tip_tables(e->dmodels.as<ModelDev>(), e->delemT.as<int32_t>(),
+                               (int)e->items.size(), e->nelemT, st));
+    const int maxgrid = e->num_cu * 8;
+    for (const Launch& L : e->fwd) {
+        TIMED(2, launch_level(false, L.S, e->ditems.as<Item>(), e->djobs.as<Job>() + L.job_begin,
+                              e->dtaskoff.as<int32_t>() + L.taskoff_begin, L.njobs,
+                              e->dparts.as<double>(), std::min(L.ntasks, maxgrid), e->tile, st));
+    }
+    if (grad)
+        for (const Launch& L : e->adj) {
+            TIMED(3, launch_level(true, L.S, e->ditems.as<Item>(), e->djobs.as<Job>() + L.job_begin,
+                                  e->dtaskoff.as<int32_t>() + L.taskoff_begin, L.njobs,
+                                  e->dparts.as<double>(), std::min(L.ntasks, maxgrid), e->tile,
+                                  st));
+        }
+    const int nops = grad ? (int)e->red.size() : e->n_ll_ops;
+    TIMED(4, launch_reduce(e->dred.as<RedOp>(), nops, e->dparts.as<double>(),
+                           e->dout.as<double>(), st));
+#undef TIMED
+    HIP_TRY(hipMemcpyAsync(e->hout.p, e->dout.p, e->total_out * sizeof(double),
+                           hipMemcpyDeviceToHost, st));
+    if (tm) HIP_TRY(hipEventRecord(e->ev_end, st));
+    return MCP_OK;
+}
+
+static int engine_eval(mcp_engine* e, const double* blv, double* ll, double* grad, bool want_grad) {
+    if (!e || !blv || !ll) return fail(MCP_ERR_INVALID, "null argument");
+    for (auto& it : e->items)
+        if (!it.model_set) return fail(MCP_ERR_STATE, "substitution model not set");
+    DeviceGuard guard(e->device);
+    std::memcpy(e->hblv.p, blv, e->total_blv * sizeof(double));
+    const int gi = want_grad ? 1 : 0;
+    if (e->use_graph) {
+        if (!e->graph_exec[gi]) {
+            hipGraph_t g = nullptr;
+            HIP_TRY(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+            int st = enqueue_eval(e, want_grad);
+            hipGraph_t cap = nullptr;
+            hipError_t ce = hipStreamEndCapture(e->stream, &cap);
+            if (st) {
+                if (cap) (void)hipGraphDestroy(cap);
+                return st;
+            }
+            if (ce != hipSuccess) return fail(MCP_ERR_HIP, "graph capture failed");
+            g = cap;
+            hipError_t ie = hipGraphInstantiate(&e->graph_exec[gi], g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (ie != hipSuccess) {
+                e->graph_exec[gi] = nullptr;
+                return fail(MCP_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(ie));
+            }
+        }
+        HIP_TRY(hipGraphLaunch(e->graph_exec[gi], e->stream));
+    } else {
+        int st = enqueue_eval(e, want_grad);
+        if (st) return st;
+    }
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    if (e->timing && !e->use_graph) {
+        float ms;
+        HIP_TRY(hipEventElapsedTime(&ms, e->ev_begin, e->ev_end));
+        e->last_ms[0] = ms;
+        for (int k = 1; k < 5; ++k) e->last_ms[k] = 0.0;
+        e->kms.assign(e->nk, 0.0);
+        for (int k = 0; k < e->nk; ++k) {
+            HIP_TRY(hipEventElapsedTime(&ms, e->kev[2 * k], e->kev[2 * k + 1]));
+            e->kms[k] = ms;
+            e->last_ms[e->kphase[k]] += ms;
+        }
+    }
+    const double* o = e->hout.as<double>();
+    size_t goff = 0;
+    for (size_t i = 0; i < e->items.size(); ++i) {
+        const ItemHost& it = e->items[i];
+        ll[i] = o[it.out_off];
+        if (want_grad && grad) std::memcpy(grad + goff, o + it.out_off + 1, (it.N - 1) * sizeof(double));
+        goff += it.N - 1;
+    }
+    return MCP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+
+extern "C" {
+
+const char* mcp_last_error(void) { return g_last_error.c_str(); }
+
+int mcp_plan_create(mcp_plan** out, const mcp_plan_desc* d) {
+    if (!out || !d) return fail(MCP_ERR_INVALID, "null argument");
+    if (!d->parent || !d->tip_partials || d->n_nodes < 2 || d->n_states < 1 || d->n_patterns < 1 ||
+        d->n_rates < 1)
+        return fail(MCP_ERR_INVALID, "plan: bad sizes or null pointers");
+    const int N = d->n_nodes, S = d->n_states, C = d->n_patterns;
+    std::vector<char> has_child(N, 0);
+    for (int v = 0; v < N; ++v) {
+        const int p = d->parent[v];
+        if (p < 0 || p > N) return fail(MCP_ERR_INVALID, "parent num out of range");
+        if (p > 0) has_child[p - 1] = 1;
+    }
+    std::vector<int32_t> leaves;
+    for (int v = 0; v < N; ++v)
+        if (!has_child[v]) leaves.push_back(v + 1);
+    std::vector<double> tips((size_t)S * C * leaves.size());
+    for (size_t j = 0; j < leaves.size(); ++j)
+        std::memcpy(tips.data() + (size_t)S * C * j,
+                    d->tip_partials + (size_t)S * C * (leaves[j] - 1), sizeof(double) * S * C);
+    mcp_dataset_desc ds;
+    ds.n_states = S;
+    ds.n_patterns = C;
+    ds.n_leaves = (int32_t)leaves.size();
+    ds.leaf_nums = leaves.data();
+    ds.tip_partials = tips.data();
+    ds.pattern_weights = d->pattern_weights;
+    ds.flags = d->flags;
+    mcp_item_desc it;
+    it.dataset = 0;
+    it.n_nodes = N;
+    it.n_rates = d->n_rates;
+    it.parent = d->parent;
+    mcp_batch_desc bd;
+    bd.n_datasets = 1;
+    bd.datasets = &ds;
+    bd.n_items = 1;
+    bd.items = &it;
+    bd.device = d->device;
+    bd.tile = 0;
+    return engine_create(out, &bd);
+}
+
+void mcp_plan_destroy(mcp_plan* p) { delete p; }
+
+int mcp_set_model(mcp_plan* p, const double* U, const double* D, const double* Uinv, double mu,
+                  const double* rates, const double* pi) {
+    return engine_set_model(p, 0, U, D, Uinv, mu, rates, pi);
+}
+
+int mcp_set_topology(mcp_plan* p, const int32_t* parent) { return engine_set_topology(p, 0, parent); }
+
+int mcp_logl(mcp_plan* p, const double* blv, double* ll) {
+    if (p && p->items.size() != 1) return fail(MCP_ERR_INVALID, "handle is a batch");
+    return engine_eval(p, blv, ll, nullptr, false);
+}
+
+int mcp_logl_grad(mcp_plan* p, const double* blv, double* ll, double* grad) {
+    if (p && p->items.size() != 1) return fail(MCP_ERR_INVALID, "handle is a batch");
+    if (!grad) return fail(MCP_ERR_INVALID, "null gradient");
+    return engine_eval(p, blv, ll, grad, true);
+}
+
+int mcp_batch_create(mcp_batch** out, const mcp_batch_desc* d) { return engine_create(out, d); }
+
+void mcp_batch_destroy(mcp_batch* b) { delete b; }
+
+int mcp_batch_set_model(mcp_batch* b, int32_t item, const double* U, const double* D,
+                        const double* Uinv, double mu, const double* rates, const double* pi) {
+    return engine_set_model(b, item, U, D, Uinv, mu, rates, pi);
+}
+
+int mcp_batch_set_topology(mcp_batch* b, int32_t item, const int32_t* parent) {
+    return engine_set_topology(b, item, parent);
+}
+
+int mcp_batch_logl(mcp_batch* b, const double* blv, double* ll) {
+    return engine_eval(b, blv, ll, nullptr, false);
+}
+
+int mcp_batch_logl_grad(mcp_batch* b, const double* blv, double* ll, double* grad) {
+    if (!grad) return fail(MCP_ERR_INVALID, "null gradient");
+    return engine_eval(b, blv, ll, grad, true);
+}
+
+int mcp_use_graph(mcp_batch* b, int32_t on) {
+    if (!b) return fail(MCP_ERR_INVALID, "null handle");
+    DeviceGuard guard(b->device);
+    if (b->stream) HIP_TRY(hipStreamSynchronize(b->stream));
+    b->use_graph = on != 0;
+    if (!b->use_graph) b->drop_graphs();
+    return MCP_OK;
+}
+
+int mcp_set_timing(mcp_batch* b, int32_t on) {
+    if (!b) return fail(MCP_ERR_INVALID, "null handle");
+    DeviceGuard guard(b->device);
+    if (on && !b->ev_begin) {
+        HIP_TRY(hipEventCreate(&b->ev_begin));
+        HIP_TRY(hipEventCreate(&b->ev_end));
+    }
+    b->timing = on != 0;
+    return MCP_OK;
+}
+
+int mcp_get_timing(mcp_batch* b, double* ms5) {
+    if (!b || !ms5) return fail(MCP_ERR_INVALID, "null argument");
+    for (int k = 0; k < 5; ++k) ms5[k] = b->last_ms[k];
+    return MCP_OK;
+}
+
+int mcp_get_launch_times(mcp_batch* b, double* ms, int32_t* phase, int32_t cap) {
+    if (!b) return -fail(MCP_ERR_INVALID, "null handle");
+    const int n = std::min<int>(cap, (int)b->kms.size());
+    for (int k = 0; k < n; ++k) {
+        if (ms) ms[k] = b->kms[k];
+        if (phase) phase[k] = b->kphase[k];
+    }
+    return (int)b->kms.size();
+}
+
+int mcp_get_bytes(mcp_batch* b, double* bytes2) {
+    if (!b || !bytes2) return fail(MCP_ERR_INVALID, "null argument");
+    bytes2[0] = b->bytes_fwd;
+    bytes2[1] = b->bytes_adj;
+    return MCP_OK;
+}
+
+int mcp_get_sizes(mcp_batch* b, int64_t* sizes2) {
+    if (!b || !sizes2) return fail(MCP_ERR_INVALID, "null argument");
+    sizes2[0] = (int64_t)b->items.size();
+    sizes2[1] = (int64_t)b->total_patterns;
+    return MCP_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,238 @@
+"""Python handles over the C ABI: `Plan` (one tree) and `Batch` (many
+(tree, chain) items of uneven size).  Device memory, level schedules and the
+evaluation sequence live in the native library; these classes only marshal
+host arrays."""
+
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, dptr, iptr
+
+__all__ = ["Plan", "Batch", "Model"]
+
+
+class Model:
+    """Eigensystem of Q plus category rates and base frequencies."""
+
+    __slots__ = ("U", "D", "Uinv", "mu", "rates", "pi")
+
+    def __init__(self, U, D, Uinv, mu, rates, pi):
+        self.U = np.asfortranarray(U, dtype=np.float64)
+        self.D = np.ascontiguousarray(D, dtype=np.float64)
+        self.Uinv = np.asfortranarray(Uinv, dtype=np.float64)
+        self.mu = float(mu)
+        self.rates = np.ascontiguousarray(np.atleast_1d(rates), dtype=np.float64)
+        self.pi = np.ascontiguousarray(pi, dtype=np.float64)
+        S = self.D.shape[0]
+        if self.U.shape != (S, S) or self.Uinv.shape != (S, S) or self.pi.shape != (S,):
+            raise ValueError("model arrays have inconsistent shapes")
+
+    def key(self) -> bytes:
+        return (self.U.tobytes("F") + self.D.tobytes() + self.Uinv.tobytes("F") +
+                np.float64(self.mu).tobytes() + self.rates.tobytes() + self.pi.tobytes())
+
+
+class _Handle:
+    _destroy = None
+
+    def __init__(self):
+        self._h = ctypes.c_void_p()
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self):
+        if self._h and self._h.value:
+            getattr(_lib.load_library(), self._destroy)(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def use_graph(self, on: bool = True) -> None:
+        check(_lib.load_library().mcp_use_graph(self._h, 1 if on else 0))
+
+    def set_timing(self, on: bool = True) -> None:
+        check(_lib.load_library().mcp_set_timing(self._h, 1 if on else 0))
+
+    def timing(self) -> np.ndarray:
+        """ms of the last evaluation: [total, P build, post-order, pre-order, reduce]."""
+        out = np.zeros(5)
+        check(_lib.load_library().mcp_get_timing(self._h, dptr(out)))
+        return out
+
+    def launch_times(self):
+        """(ms, phase) of every kernel launch of the last timed evaluation."""
+        lib = _lib.load_library()
+        n = lib.mcp_get_launch_times(self._h, None, None, 0)
+        if n < 0:
+            check(-n)
+        ms = np.zeros(max(n, 1))
+        ph = np.zeros(max(n, 1), dtype=np.int32)
+        lib.mcp_get_launch_times(self._h, dptr(ms), iptr(ph), n)
+        return ms[:n], ph[:n]
+
+    def algorithmic_bytes(self) -> np.ndarray:
+        out = np.zeros(2)
+        check(_lib.load_library().mcp_get_bytes(self._h, dptr(out)))
+        return out
+
+    def sizes(self) -> tuple[int, int]:
+        out = np.zeros(2, dtype=np.int64)
+        check(_lib.load_library().mcp_get_sizes(
+            self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))))
+        return int(out[0]), int(out[1])
+
+
+class Plan(_Handle):
+    """Device plan for one tree (the PhyloDist drop-in).
+
+    parent:  int32[N] parent num of node num i+1 (0 for the root, num N)
+    x:       tip partials (S, C, N), reference layout (Fortran order)
+    weights: optional pattern weights [C]
+    """
+
+    _destroy = "mcp_plan_destroy"
+
+    def __init__(self, parent, x, n_rates: int, weights=None, device: int = 0,
+                 compress: bool = False):
+        super().__init__()
+        self.parent = np.ascontiguousarray(parent, dtype=np.int32)
+        x = np.asfortranarray(x, dtype=np.float64)
+        S, C, N = x.shape
+        if N != self.parent.shape[0]:
+            raise _lib.McpDimensionMismatch(_lib.MCP_ERR_DIMENSION,
+                                            "x and the tree disagree on the number of nodes")
+        w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float64)
+        if w is not None and w.shape != (C,):
+            raise _lib.McpDimensionMismatch(_lib.MCP_ERR_DIMENSION, "weights must have length C")
+        self.N, self.S, self.C, self.R = N, S, C, int(n_rates)
+        d = _lib.PlanDesc(N, S, C, self.R, iptr(self.parent), dptr(x), dptr(w), int(device),
+                          _lib.MCP_FLAG_COMPRESS_PATTERNS if compress else 0)
+        lib = _lib.load_library()
+        check(lib.mcp_plan_create(ctypes.byref(self._h), ctypes.byref(d)))
+        self._model_key = None
+
+    def set_model(self, m: Model) -> None:
+        k = m.key()
+        if k == self._model_key:
+            return
+        if m.D.shape[0] != self.S or m.rates.shape[0] != self.R:
+            raise _lib.McpDimensionMismatch(_lib.MCP_ERR_DIMENSION,
+                                            "model does not match the plan's S or R")
+        check(_lib.load_library().mcp_set_model(self._h, dptr(m.U), dptr(m.D), dptr(m.Uinv), m.mu,
+                                                dptr(m.rates), dptr(m.pi)))
+        self._model_key = k
+
+    def set_topology(self, parent) -> None:
+        p = np.ascontiguousarray(parent, dtype=np.int32)
+        check(_lib.load_library().mcp_set_topology(self._h, iptr(p)))
+        self.parent = p
+
+    def logl(self, blv) -> float:
+        b = np.ascontiguousarray(blv, dtype=np.float64)
+        if b.shape != (self.N - 1,):
+            raise _lib.McpDimensionMismatch(_lib.MCP_ERR_DIMENSION, "blv must have length N-1")
+        out = np.zeros(1)
+        check(_lib.load_library().mcp_logl(self._h, dptr(b), dptr(out)))
+        return float(out[0])
+
+    def logl_grad(self, blv):
+        b = np.ascontiguousarray(blv, dtype=np.float64)
+        if b.shape != (self.N - 1,):
+            raise _lib.McpDimensionMismatch(_lib.MCP_ERR_DIMENSION, "blv must have length N-1")
+        out = np.zeros(1)
+        g = np.zeros(self.N - 1)
+        check(_lib.load_library().mcp_logl_grad(self._h, dptr(b), dptr(out), dptr(g)))
+        return float(out[0]), g
+
+
+class Batch(_Handle):
+    """Ragged batch of items.
+
+    datasets: list of dicts {x_leaves (S, C, L) Fortran, leaf_nums [L], weights or None,
+              compress bool}
+    items:    list of dicts {dataset int, parent int32[N], n_rates int}
+    """
+
+    _destroy = "mcp_batch_destroy"
+
+    def __init__(self, datasets, items, device: int = 0, tile: int = 0):
+        super().__init__()
+        self._keep = []
+        dsd = (_lib.DatasetDesc * len(datasets))()
+        for k, d in enumerate(datasets):
+            x = np.asfortranarray(d["x_leaves"], dtype=np.float64)
+            leaf = np.ascontiguousarray(d["leaf_nums"], dtype=np.int32)
+            w = d.get("weights")
+            w = None if w is None else np.ascontiguousarray(w, dtype=np.float64)
+            S, C, L = x.shape
+            if leaf.shape != (L,):
+                raise _lib.McpDimensionMismatch(_lib.MCP_ERR_DIMENSION, "leaf_nums length")
+            self._keep += [x, leaf, w]
+            dsd[k] = _lib.DatasetDesc(S, C, L, iptr(leaf), dptr(x), dptr(w),
+                                      _lib.MCP_FLAG_COMPRESS_PATTERNS if d.get("compress") else 0)
+        itd = (_lib.ItemDesc * len(items))()
+        self.item_nodes = []
+        self.item_S = []
+        self.item_R = []
+        for i, it in enumerate(items):
+            p = np.ascontiguousarray(it["parent"], dtype=np.int32)
+            self._keep.append(p)
+            itd[i] = _lib.ItemDesc(int(it["dataset"]), p.shape[0], int(it["n_rates"]), iptr(p))
+            self.item_nodes.append(p.shape[0])
+            self.item_S.append(np.asarray(datasets[int(it["dataset"])]["x_leaves"]).shape[0])
+            self.item_R.append(int(it["n_rates"]))
+        bd = _lib.BatchDesc(len(datasets), dsd, len(items), itd, int(device), int(tile))
+        check(_lib.load_library().mcp_batch_create(ctypes.byref(self._h), ctypes.byref(bd)))
+        self._keep = []
+        self.n_items = len(items)
+        self.blv_off = np.concatenate([[0], np.cumsum([n - 1 for n in self.item_nodes])])
+        self._model_keys = [None] * self.n_items
+
+    def set_model(self, item: int, m: Model) -> None:
+        k = m.key()
+        if k == self._model_keys[item]:
+            return
+        if m.D.shape[0] != self.item_S[item] or m.rates.shape[0] != self.item_R[item]:
+            raise _lib.McpDimensionMismatch(_lib.MCP_ERR_DIMENSION,
+                                            "model does not match the item's S or R")
+        check(_lib.load_library().mcp_batch_set_model(self._h, int(item), dptr(m.U), dptr(m.D),
+                                                      dptr(m.Uinv), m.mu, dptr(m.rates),
+                                                      dptr(m.pi)))
+        self._model_keys[item] = k
+
+    def set_topology(self, item: int, parent) -> None:
+        p = np.ascontiguousarray(parent, dtype=np.int32)
+        check(_lib.load_library().mcp_batch_set_topology(self._h, int(item), iptr(p)))
+
+    def _blv(self, blv_concat):
+        b = np.ascontiguousarray(blv_concat, dtype=np.float64)
+        if b.shape != (int(self.blv_off[-1]),):
+            raise _lib.McpDimensionMismatch(_lib.MCP_ERR_DIMENSION,
+                                            "blv_concat has the wrong length")
+        return b
+
+    def logl(self, blv_concat) -> np.ndarray:
+        b = self._blv(blv_concat)
+        ll = np.zeros(self.n_items)
+        check(_lib.load_library().mcp_batch_logl(self._h, dptr(b), dptr(ll)))
+        return ll
+
+    def logl_grad(self, blv_concat):
+        b = self._blv(blv_concat)
+        ll = np.zeros(self.n_items)
+        g = np.zeros(b.shape[0])
+        check(_lib.load_library().mcp_batch_logl_grad(self._h, dptr(b), dptr(ll), dptr(g)))
+        return ll, g
+
+    def split(self, grad_concat):
+        return [grad_concat[self.blv_off[i]:self.blv_off[i + 1]] for i in range(self.n_items)]
