@@ -33,6 +33,8 @@ def log(msg):
 
 
 def load_package():
+    if "mcphylo_amd" in sys.modules:
+        return sys.modules["mcphylo_amd"]
     pkg = os.path.join(ROOT, "mcphylo.jl_amd")
     spec = importlib.util.spec_from_file_location(
         "mcphylo_amd", os.path.join(pkg, "__init__.py"), submodule_search_locations=[pkg])
@@ -40,6 +42,74 @@ def load_package():
     sys.modules["mcphylo_amd"] = mod
     spec.loader.exec_module(mod)
     return mod
+
+
+# loaded at import so that spawned workers can unpickle the generators
+mcp = load_package()
+
+
+def make_batch_data(n_trees):
+    """Ragged batch inputs, generated in worker processes started with
+    'spawn' before this process touches the GPU."""
+    import multiprocessing as mp
+    from concurrent.futures import ProcessPoolExecutor
+    if mcp._lib._lib is not None:
+        raise RuntimeError("batch data must be generated before the HIP library is loaded")
+    workers = max(1, min(8, (os.cpu_count() or 1)))
+    with ProcessPoolExecutor(workers, mp_context=mp.get_context("spawn")) as pool:
+        return mcp.synthetic.make_batched_workload(n_trees=n_trees, pool=pool)
+
+
+def bench_batched(data, steps, warmup, sample_trees):
+    """Batched logL+grad over all (tree, chain) items in one engine, and the
+    plain single-tree launch sequence on a sample of the same trees."""
+    datasets, items, models, blv = data
+    b = mcp.Batch(datasets, items)
+    for i, m in enumerate(models):
+        b.set_model(i, mcp.Model(*m))
+    n_items, total_patterns = b.sizes()
+    for _ in range(warmup):
+        b.logl_grad(blv)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        b.logl_grad(blv)
+    wall = time.perf_counter() - t0
+    b.set_timing(True)
+    b.logl_grad(blv)
+    ms, ph = b.launch_times()
+    b.set_timing(False)
+    bytes_fwd, bytes_adj = b.algorithmic_bytes()
+    fwd_ms, adj_ms = float(np.sum(ms[ph == 2])), float(np.sum(ms[ph == 3]))
+    b.close()
+    out = {"items": n_items, "patterns_per_step": total_patterns,
+           "ms_per_step": 1e3 * wall / steps,
+           "value": total_patterns * steps / wall,
+           "post_order_GBs": bytes_fwd / (fwd_ms * 1e-3) / 1e9,
+           "pre_order_GBs": bytes_adj / (adj_ms * 1e-3) / 1e9,
+           "post_order_ms": fwd_ms, "pre_order_ms": adj_ms,
+           "launches": int(ms.shape[0])}
+    # plain single-tree sequence on the first chain of the first trees
+    pat, tsum = 0, 0.0
+    offs = np.concatenate([[0], np.cumsum([it["parent"].shape[0] - 1 for it in items])])
+    chains = len(items) // len(datasets)
+    for t in range(min(sample_trees, len(datasets))):
+        i = t * chains
+        d = datasets[t]
+        L, C = d["tip_codes"].shape
+        pb = mcp.Batch([d], [dict(items[i], dataset=0)])
+        pb.set_model(0, mcp.Model(*models[i]))
+        bl = blv[offs[i]:offs[i + 1]]
+        pb.logl_grad(bl)
+        t1 = time.perf_counter()
+        for _ in range(3):
+            pb.logl_grad(bl)
+        tsum += (time.perf_counter() - t1) / 3
+        pat += C
+        pb.close()
+    out["plain_value"] = pat / tsum
+    out["plain_sample_trees"] = int(min(sample_trees, len(datasets)))
+    out["batched_over_plain"] = out["value"] / out["plain_value"]
+    return out
 
 
 WORKLOADS = {
@@ -77,6 +147,11 @@ def main():
     ap.add_argument("--workload", default="gtr100", choices=sorted(WORKLOADS))
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--batch-trees", type=int, default=256,
+                    help="trees of the ragged batch (4 chains each); 0 skips it.  The batch "
+                         "is generated in spawned worker processes before the GPU is touched; "
+                         "use 0 under a profiler that initialises the GPU at load")
+    ap.add_argument("--batch-steps", type=int, default=5)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -84,7 +159,10 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
-    mcp = load_package()
+    batch_data = None
+    if args.batch_trees > 0:
+        log(f"generating ragged batch ({args.batch_trees} trees x 4 chains)")
+        batch_data = make_batch_data(args.batch_trees)
     if mcp._lib.device_count() < 1:
         log("no HIP device visible")
         sys.exit(1)
@@ -164,6 +242,13 @@ def main():
                               "reduce": float(np.sum(phases[ph == 4])) / args.steps},
         "algorithmic_bytes_per_step": {"post_order": bytes_fwd, "pre_order": bytes_adj},
     }
+    if batch_data is not None:
+        log("timing ragged batch")
+        result["batched"] = bench_batched(batch_data, args.batch_steps, 2, 16)
+        result["batched"]["workload"] = (
+            f"{args.batch_trees} trees, taxa ~ U{{16..512}}, patterns ~ LogU[500, 20000], "
+            "restriction/GTR-4 50/50, 4 gamma cats, 4 chains per tree, FP64 logL+grad")
+        log(f"batched: {result['batched']['ms_per_step']:.2f} ms/step")
     if not args.no_cpu:
         log("timing CPU baseline")
         result["cpu_baseline"] = cpu_baseline(mcp, w, blv, args.cpu_seconds)

@@ -96,9 +96,15 @@ typedef struct mcp_dataset_desc {
     int32_t n_patterns;
     int32_t n_leaves;
     const int32_t* leaf_nums;       /* [n_leaves] node nums of the leaves */
-    const double* tip_partials;     /* [S*C*n_leaves] x[s,c,j] for leaf leaf_nums[j] */
+    const double* tip_partials;     /* [S*C*n_leaves] x[s,c,j] for leaf leaf_nums[j], or NULL */
     const double* pattern_weights;  /* [C] or NULL */
     uint32_t flags;                 /* MCP_FLAG_* */
+    /* Alternative to tip_partials (used when tip_partials is NULL): state
+     * codes tip_codes[j*C + c] indexing rows of code_table [n_codes x S]
+     * (e.g. S one-hot rows plus an all-ones row for gaps). */
+    const uint8_t* tip_codes;
+    int32_t n_codes;
+    const double* code_table;
 } mcp_dataset_desc;
 
 typedef struct mcp_item_desc {

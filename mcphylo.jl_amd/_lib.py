@@ -69,6 +69,8 @@ class DatasetDesc(ctypes.Structure):
         ("n_states", ctypes.c_int32), ("n_patterns", ctypes.c_int32),
         ("n_leaves", ctypes.c_int32), ("leaf_nums", _ip), ("tip_partials", _dp),
         ("pattern_weights", _dp), ("flags", ctypes.c_uint32),
+        ("tip_codes", ctypes.POINTER(ctypes.c_uint8)), ("n_codes", ctypes.c_int32),
+        ("code_table", _dp),
     ]
 
 

@@ -228,8 +228,11 @@ struct DeviceGuard {
 // ---------------------------------------------------------------------------
 
 int build_dataset(const mcp_dataset_desc& d, int tile, DatasetHost& out) {
-    if (d.n_states < 1 || d.n_patterns < 1 || d.n_leaves < 1 || !d.leaf_nums || !d.tip_partials)
+    if (d.n_states < 1 || d.n_patterns < 1 || d.n_leaves < 1 || !d.leaf_nums ||
+        (!d.tip_partials && (!d.tip_codes || !d.code_table || d.n_codes < 1)))
         return fail(MCP_ERR_INVALID, "dataset: sizes must be positive and pointers non-null");
+    if (!d.tip_partials && d.n_codes > 256)
+        return fail(MCP_ERR_UNSUPPORTED, "dataset: more than 256 tip codes");
     if (!supported_states(d.n_states))
         return fail(MCP_ERR_UNSUPPORTED, "dataset: no kernel built for " +
                                              std::to_string(d.n_states) + " states");
@@ -255,7 +258,19 @@ int build_dataset(const mcp_dataset_desc& d, int tile, DatasetHost& out) {
     std::unordered_map<std::string, int> code_of;
     std::vector<uint8_t> codes((size_t)L * C);
     out.tiptab.clear();
-    for (int j = 0; j < L; ++j) {
+    if (!d.tip_partials) {
+        for (int j = 0; j < L; ++j) {
+            const uint8_t* src = d.tip_codes + (size_t)order[j] * C;
+            for (int c = 0; c < C; ++c) {
+                if (src[c] >= d.n_codes)
+                    return fail(MCP_ERR_INVALID, "dataset: tip code out of range");
+                codes[(size_t)j * C + c] = src[c];
+            }
+        }
+        out.tiptab.assign(d.code_table, d.code_table + (size_t)d.n_codes * S);
+        out.K = d.n_codes;
+    }
+    for (int j = 0; j < L && d.tip_partials; ++j) {
         const double* xl = d.tip_partials + (size_t)S * C * order[j];
         for (int c = 0; c < C; ++c) {
             std::string key(reinterpret_cast<const char*>(xl + (size_t)S * c), sizeof(double) * S);
@@ -274,7 +289,7 @@ int build_dataset(const mcp_dataset_desc& d, int tile, DatasetHost& out) {
             codes[(size_t)j * C + c] = (uint8_t)code;
         }
     }
-    out.K = (int)code_of.size();
+    if (d.tip_partials) out.K = (int)code_of.size();
 
     std::vector<double> w(C, 1.0);
     if (d.pattern_weights) w.assign(d.pattern_weights, d.pattern_weights + C);
@@ -934,6 +949,9 @@ int mcp_plan_create(mcp_plan** out, const mcp_plan_desc* d) {
     ds.tip_partials = tips.data();
     ds.pattern_weights = d->pattern_weights;
     ds.flags = d->flags;
+    ds.tip_codes = nullptr;
+    ds.n_codes = 0;
+    ds.code_table = nullptr;
     mcp_item_desc it;
     it.dataset = 0;
     it.n_nodes = N;

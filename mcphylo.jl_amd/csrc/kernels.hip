@@ -394,19 +394,33 @@ __global__ void __launch_bounds__(256) k_adj_level(const Item* __restrict__ item
 }
 
 // ---------------------------------------------------------------------------
-// final sums: one wavefront per output, fixed order
+// final sums: one block per output, fixed order.  Each thread sums a
+// strided slice with 8 independent accumulators (the loads of an unrolled
+// step are independent, so long ranges are not latency-bound), then the
+// block reduces in a fixed tree.
 // ---------------------------------------------------------------------------
 
-__global__ void k_reduce(const RedOp* __restrict__ ops, int nops, const double* __restrict__ parts,
-                         double* __restrict__ out) {
-    const int wave = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
-    const int lane = threadIdx.x & 63;
-    if (wave >= nops) return;
-    const RedOp op = ops[wave];
-    double s = 0.0;
-    for (int i = lane; i < op.len; i += 64) s += parts[op.start + i];
-    s = wave_sum(s);
-    if (lane == 0) out[op.dst] = s;
+__global__ void __launch_bounds__(256) k_reduce(const RedOp* __restrict__ ops, int nops,
+                                                const double* __restrict__ parts,
+                                                double* __restrict__ out) {
+    __shared__ double red[16];
+    const RedOp op = ops[blockIdx.x];
+    const int nt = blockDim.x;
+    const double* p = parts + op.start;
+    double a[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) a[k] = 0.0;
+    int i = threadIdx.x;
+    for (; i + 7 * nt < op.len; i += 8 * nt) {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) a[k] += p[i + k * nt];
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+        if (i + k * nt < op.len) a[k] += p[i + k * nt];
+    const double s = ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
+    const double tot = block_sum(s, red);
+    if (threadIdx.x == 0) out[op.dst] = tot;
 }
 
 // ---------------------------------------------------------------------------
@@ -466,10 +480,7 @@ hipError_t launch_level(bool adjoint, int S, const Item* items, const Job* jobs,
 hipError_t launch_reduce(const RedOp* ops, int nops, const double* parts, double* out,
                          hipStream_t st) {
     if (nops <= 0) return hipSuccess;
-    const int tpb = 256;
-    const int waves_per_block = tpb / 64;
-    hipLaunchKernelGGL(k_reduce, dim3((nops + waves_per_block - 1) / waves_per_block), dim3(tpb),
-                       0, st, ops, nops, parts, out);
+    hipLaunchKernelGGL(k_reduce, dim3(nops), dim3(256), 0, st, ops, nops, parts, out);
     return hipGetLastError();
 }
 

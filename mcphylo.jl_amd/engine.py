@@ -158,8 +158,9 @@ class Plan(_Handle):
 class Batch(_Handle):
     """Ragged batch of items.
 
-    datasets: list of dicts {x_leaves (S, C, L) Fortran, leaf_nums [L], weights or None,
-              compress bool}
+    datasets: list of dicts {leaf_nums [L], weights or None, compress bool} plus either
+              x_leaves (S, C, L) tip partials, or tip_codes uint8 (L, C) with
+              code_table (K, S)
     items:    list of dicts {dataset int, parent int32[N], n_rates int}
     """
 
@@ -169,17 +170,30 @@ class Batch(_Handle):
         super().__init__()
         self._keep = []
         dsd = (_lib.DatasetDesc * len(datasets))()
+        self.dataset_S = []
         for k, d in enumerate(datasets):
-            x = np.asfortranarray(d["x_leaves"], dtype=np.float64)
             leaf = np.ascontiguousarray(d["leaf_nums"], dtype=np.int32)
             w = d.get("weights")
             w = None if w is None else np.ascontiguousarray(w, dtype=np.float64)
-            S, C, L = x.shape
+            flags = _lib.MCP_FLAG_COMPRESS_PATTERNS if d.get("compress") else 0
+            if d.get("x_leaves") is not None:
+                x = np.asfortranarray(d["x_leaves"], dtype=np.float64)
+                S, C, L = x.shape
+                codes = table = None
+                K = 0
+            else:
+                codes = np.ascontiguousarray(d["tip_codes"], dtype=np.uint8)
+                table = np.ascontiguousarray(d["code_table"], dtype=np.float64)
+                L, C = codes.shape
+                K, S = table.shape
+                x = None
             if leaf.shape != (L,):
                 raise _lib.McpDimensionMismatch(_lib.MCP_ERR_DIMENSION, "leaf_nums length")
-            self._keep += [x, leaf, w]
-            dsd[k] = _lib.DatasetDesc(S, C, L, iptr(leaf), dptr(x), dptr(w),
-                                      _lib.MCP_FLAG_COMPRESS_PATTERNS if d.get("compress") else 0)
+            self._keep += [x, leaf, w, codes, table]
+            cp = None if codes is None else codes.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))
+            dsd[k] = _lib.DatasetDesc(S, C, L, iptr(leaf), dptr(x), dptr(w), flags, cp, K,
+                                      dptr(table))
+            self.dataset_S.append(S)
         itd = (_lib.ItemDesc * len(items))()
         self.item_nodes = []
         self.item_S = []
@@ -189,7 +203,7 @@ class Batch(_Handle):
             self._keep.append(p)
             itd[i] = _lib.ItemDesc(int(it["dataset"]), p.shape[0], int(it["n_rates"]), iptr(p))
             self.item_nodes.append(p.shape[0])
-            self.item_S.append(np.asarray(datasets[int(it["dataset"])]["x_leaves"]).shape[0])
+            self.item_S.append(self.dataset_S[int(it["dataset"])])
             self.item_R.append(int(it["n_rates"]))
         bd = _lib.BatchDesc(len(datasets), dsd, len(items), itd, int(device), int(tile))
         check(_lib.load_library().mcp_batch_create(ctypes.byref(self._h), ctypes.byref(bd)))

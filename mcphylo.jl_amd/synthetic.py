@@ -236,3 +236,105 @@ def make_workload(name: str, seed: int | None = None, **kw) -> Workload:
         x = simulate_alignment(t, model_fn, pi, r, rates, C, rng)
         return Workload(t, x, None, model_fn, pi, r, rates)
     raise ValueError(f"unknown workload {name!r}")
+
+
+# ---------------------------------------------------------------------------
+# ragged batches (many trees x chains), generated as flat arrays
+# ---------------------------------------------------------------------------
+
+def coalescent_parent(n: int, rng):
+    """Random rooted binary topology by pairwise coalescence as a parent
+    array: leaves 1..n, internal nodes n+1..2n-1 in merge order (children
+    before parents), root 2n-1."""
+    N = 2 * n - 1
+    parent = np.zeros(N, dtype=np.int32)
+    pool = list(range(1, n + 1))
+    nxt = n + 1
+    while len(pool) > 1:
+        i, j = rng.choice(len(pool), size=2, replace=False)
+        a, b = pool[i], pool[j]
+        for k in sorted((i, j), reverse=True):
+            pool.pop(k)
+        parent[a - 1] = nxt
+        parent[b - 1] = nxt
+        pool.append(nxt)
+        nxt += 1
+    return parent
+
+
+def simulate_codes(parent, blv, U, D, Uinv, mu, rates, pi, C: int, rng) -> np.ndarray:
+    """Leaf state codes (n_leaves x C, uint8, leaves in increasing num order)
+    simulated down the tree; the category of a site is uniform over R."""
+    N = parent.shape[0]
+    S = len(pi)
+    rates = np.atleast_1d(rates)
+    R = rates.shape[0]
+    cat = rng.integers(R, size=C)
+    # parents before children: order by depth
+    depth = np.zeros(N, dtype=np.int64)
+    order = [N - 1]
+    kids = [[] for _ in range(N)]
+    for v in range(N - 1):
+        kids[parent[v] - 1].append(v)
+    k = 0
+    while k < len(order):
+        order.extend(kids[order[k]])
+        k += 1
+    state = np.zeros((N, C), dtype=np.int64)
+    cdf_pi = np.cumsum(pi)
+    state[N - 1] = np.minimum(np.searchsorted(cdf_pi, rng.random(C) * cdf_pi[-1]), S - 1)
+    for v in order[1:]:
+        E = np.exp(mu * blv[v] * D[None, :] * rates[:, None])         # (R, S)
+        P = np.clip((U[None, :, :] * E[:, None, :]) @ Uinv, 0, None)      # (R, S, S)
+        cdf = np.cumsum(P, axis=2)
+        rows = cdf[cat, state[parent[v] - 1]]                            # (C, S)
+        u = rng.random(C) * rows[:, -1]
+        state[v] = np.minimum((rows < u[:, None]).sum(axis=1), S - 1)
+    leaves = np.array([v for v in range(N) if not kids[v]])
+    return state[leaves].astype(np.uint8), leaves + 1
+
+
+def _batched_tree(i: int, seed: int, n_range, c_range, R: int, chains: int):
+    """Tree i of a ragged batch (see make_batched_workload)."""
+    rates = discrete_gamma_rates(0.5, 0.5, R) if R > 1 else np.array([1.0])
+    rng = np.random.default_rng(seed + i)
+    n = int(rng.integers(n_range[0], n_range[1] + 1))
+    C = int(round(np.exp(rng.uniform(np.log(c_range[0]), np.log(c_range[1])))))
+    if rng.random() < 0.5:
+        pi = rng.dirichlet([5.0, 5.0])
+        U, D, Ui, mu = Restriction(pi)
+    else:
+        pi, r = random_gtr(4, rng, 1.0, 5.0)
+        U, D, Ui, mu = GTR(pi, r)
+    S = len(pi)
+    parent = coalescent_parent(n, rng)
+    blv = rng.exponential(0.05, 2 * n - 2)
+    codes, leaf_nums = simulate_codes(parent, blv, U, D, Ui, mu, rates, pi, C, rng)
+    table = np.vstack([np.eye(S), np.ones((1, S))])
+    ds = {"leaf_nums": leaf_nums, "tip_codes": codes, "code_table": table, "weights": None}
+    model = (U, D, Ui, mu, rates, pi)
+    blvs = [blv * np.exp(rng.normal(0.0, 0.1, blv.shape)) for _ in range(chains)]
+    return ds, parent, model, blvs
+
+
+def make_batched_workload(n_trees: int = 256, chains: int = 4, seed: int = BASE_SEED,
+                          n_range=(16, 512), c_range=(500, 20000), R: int = 4, pool=None):
+    """Uneven batch: tree i (seed+i) has n ~ U{n_range}, C = round(exp(U[ln c_range]))
+    patterns simulated down the tree, a restriction-site (pi ~ Dir(5,5)) or
+    4-state GTR model (50/50), R gamma categories (alpha 0.5), t ~ Exp(0.05);
+    `chains` items per tree whose branch lengths are the tree's times
+    exp(N(0, 0.1^2)).  `pool` (an Executor) generates trees in parallel.
+    Returns (datasets, items, models, blv_concat) for engine.Batch."""
+    args = [(i, seed, n_range, c_range, R, chains) for i in range(n_trees)]
+    if pool is None:
+        res = [_batched_tree(*a) for a in args]
+    else:
+        res = list(pool.map(_batched_tree, *zip(*args)))
+    datasets, items, models, blvs = [], [], [], []
+    for i, (ds, parent, model, bl) in enumerate(res):
+        datasets.append(ds)
+        for b in bl:
+            items.append({"dataset": i, "parent": parent, "n_rates": R})
+            models.append(model)
+            blvs.append(b)
+    return datasets, items, models, np.concatenate(blvs)
