@@ -120,7 +120,8 @@ typedef struct mcp_batch_desc {
     int32_t n_items;
     const mcp_item_desc* items;
     int32_t device;
-    int32_t tile;                   /* patterns per task: 64, 128 or 256; 0 = default */
+    int32_t block_threads;          /* threads per block of the level kernels: 64, 128
+                                       or 256; 0 = default (128) */
 } mcp_batch_desc;
 
 /* MultiplePhyloDist (Phylodist.jl:143-272) generalised to uneven trees,

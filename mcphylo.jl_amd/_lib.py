@@ -85,7 +85,7 @@ class BatchDesc(ctypes.Structure):
     _fields_ = [
         ("n_datasets", ctypes.c_int32), ("datasets", ctypes.POINTER(DatasetDesc)),
         ("n_items", ctypes.c_int32), ("items", ctypes.POINTER(ItemDesc)),
-        ("device", ctypes.c_int32), ("tile", ctypes.c_int32),
+        ("device", ctypes.c_int32), ("block_threads", ctypes.c_int32),
     ]
 
 

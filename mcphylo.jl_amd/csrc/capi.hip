@@ -26,8 +26,9 @@
 namespace mcp {
 hipError_t launch_build_P(const ModelDev*, const int32_t*, int, long long, hipStream_t);
 hipError_t launch_tip_tables(const ModelDev*, const int32_t*, int, long long, hipStream_t);
-hipError_t launch_level(bool, int, const Item*, const Job*, const int32_t*, int, double*, int, int,
+hipError_t launch_level(bool, int, const LevelJob*, const int32_t*, int, double*, int, int,
                         hipStream_t);
+int cols_per_thread(int S);
 hipError_t launch_reduce(const RedOp*, int, const double*, double*, hipStream_t);
 bool supported_states(int S);
 }  // namespace mcp
@@ -113,6 +114,19 @@ struct PinnedBuf {
     T* as() const { return static_cast<T*>(p); }
 };
 
+// Host-side record of one node of an item (index = num-1).
+struct NodeInfo {
+    int32_t first_child;   // offset into ItemHost::children
+    int32_t nchild;
+    int32_t fslot;         // partial slab index (internal non-root), else -1
+    int32_t leaf;          // row of the dataset's tip codes (leaves), else -1
+    int32_t gslot;         // pre-order slab index in ring buffer gbuf (internal non-root), else -1
+    int32_t gbuf;          // 0/1: which pre-order ring buffer (depth parity)
+    int32_t lpart;         // offset (in the item) of the log-scaler partial sums (internal), else -1
+    int32_t gpart;         // offset (in the item) of the gradient partial sums (non-root), else -1
+    int32_t is_root;
+};
+
 struct DatasetHost {
     int S = 0, C = 0, Cp = 0, K = 0, nleaves = 0;
     std::vector<int32_t> leaf_nums;          // row j of codes is leaf leaf_nums[j]
@@ -162,18 +176,17 @@ struct Launch {
 
 struct mcp_engine {
     int device = 0;
-    int tile = 128;
+    int tpb = 128;  // threads per block of the level kernels
     int num_cu = 256;
     hipStream_t stream = nullptr;
     std::vector<DatasetHost> ds;
     std::vector<ItemHost> items;
     // device arenas
     DevBuf dF, dG0, dG1, dP, ddP, dPT, ddPT, dparts, dout, dblv, dmodel, dcodes, dweights, dtiptab,
-        dnodes, dchildren, dleafof, ditems, dmodels, djobs, dtaskoff, dred, delemP, delemT;
+        dleafof, dmodels, djobs_f, djobs_a, dmore, dtaskoff, dred, delemP, delemT;
     PinnedBuf hblv, hout;
     // schedule
     std::vector<Launch> fwd, adj;
-    std::vector<Job> jobs;
     std::vector<int32_t> taskoff;
     std::vector<RedOp> red;  // ll ops first (n_items), then gradient ops
     int n_ll_ops = 0;
@@ -227,7 +240,7 @@ struct DeviceGuard {
 // datasets: tip partials -> state codes (+ optional pattern compression)
 // ---------------------------------------------------------------------------
 
-int build_dataset(const mcp_dataset_desc& d, int tile, DatasetHost& out) {
+int build_dataset(const mcp_dataset_desc& d, int tpb, DatasetHost& out) {
     if (d.n_states < 1 || d.n_patterns < 1 || d.n_leaves < 1 || !d.leaf_nums ||
         (!d.tip_partials && (!d.tip_codes || !d.code_table || d.n_codes < 1)))
         return fail(MCP_ERR_INVALID, "dataset: sizes must be positive and pointers non-null");
@@ -317,7 +330,7 @@ int build_dataset(const mcp_dataset_desc& d, int tile, DatasetHost& out) {
         w = wu;
     }
     out.C = Cu;
-    out.Cp = (int)round_up((size_t)Cu, (size_t)tile);
+    out.Cp = (int)round_up((size_t)Cu, (size_t)tpb * cols_per_thread(S));
     out.codes.assign((size_t)L * out.Cp, 0);
     for (int j = 0; j < L; ++j)
         for (int c = 0; c < Cu; ++c)
@@ -428,7 +441,7 @@ int build_topology(ItemHost& it, const DatasetHost& d, const int32_t* parent) {
     return MCP_OK;
 }
 
-int item_from_desc(const mcp_item_desc& d, const std::vector<DatasetHost>& ds, int tile,
+int item_from_desc(const mcp_item_desc& d, const std::vector<DatasetHost>& ds, int tpb,
                    ItemHost& it) {
     if (d.dataset < 0 || d.dataset >= (int)ds.size())
         return fail(MCP_ERR_INVALID, "item: dataset index out of range");
@@ -439,7 +452,7 @@ int item_from_desc(const mcp_item_desc& d, const std::vector<DatasetHost>& ds, i
     it.N = d.n_nodes;
     it.R = d.n_rates;
     it.S = D.S;
-    it.ntiles = D.Cp / tile;
+    it.ntiles = D.Cp / (tpb * cols_per_thread(D.S));
     return build_topology(it, D, d.parent);
 }
 
@@ -459,7 +472,8 @@ static int upload(mcp_engine* e, DevBuf& b, const void* src, size_t bytes) {
 
 // Device descriptors and launch schedule; called after create and after any
 // topology change.  F/P/model arenas keep their layout (sizes do not depend
-// on topology); the pre-order ring buffers may grow.
+// on topology); the pre-order ring buffers may grow, so every job record is
+// rebuilt with fresh addresses.
 static int rebuild_schedule(mcp_engine* e) {
     e->drop_graphs();
     const int nitems = (int)e->items.size();
@@ -477,36 +491,16 @@ static int rebuild_schedule(mcp_engine* e) {
     if ((st = e->dG0.ensure(std::max<size_t>(g0, 1) * sizeof(double)))) return st;
     if ((st = e->dG1.ensure(std::max<size_t>(g1, 1) * sizeof(double)))) return st;
 
-    // node/children/leaf_of tables
-    std::vector<NodeInfo> nodes;
-    std::vector<int32_t> children, leaf_of;
-    std::vector<Item> ditems(nitems);
+    // P-build records and leaf maps
+    std::vector<int32_t> leaf_of;
     std::vector<ModelDev> dmodels(nitems);
     std::vector<int32_t> elemP(nitems + 1, 0), elemT(nitems + 1, 0);
     e->bytes_fwd = e->bytes_adj = 0.0;
     for (int i = 0; i < nitems; ++i) {
         ItemHost& it = e->items[i];
         const DatasetHost& d = e->ds[it.dataset];
-        it.nodes_off = (int)nodes.size();
-        it.children_off = (int)children.size();
         it.leafof_off = (int)leaf_of.size();
-        for (const NodeInfo& n0 : it.nodes) {
-            NodeInfo n = n0;
-            if (n.lpart >= 0) n.lpart += it.parts_off;
-            if (n.gpart >= 0) n.gpart += it.parts_off;
-            nodes.push_back(n);
-            leaf_of.push_back(n0.leaf);
-        }
-        children.insert(children.end(), it.children.begin(), it.children.end());
-        Item& x = ditems[i];
-        x.S = it.S;
-        x.C = d.C;
-        x.Cp = d.Cp;
-        x.R = it.R;
-        x.K = d.K;
-        x.N = it.N;
-        x.ntiles = it.ntiles;
-        x.tile = e->tile;
+        for (const NodeInfo& n : it.nodes) leaf_of.push_back(n.leaf);
         elemP[i + 1] = elemP[i] + (it.N - 1) * it.R * it.S * it.S;
         elemT[i + 1] = elemT[i] + (it.N - 1) * it.R * d.K * it.S;
         // algorithmic bytes
@@ -528,25 +522,10 @@ static int rebuild_schedule(mcp_engine* e) {
             }
         }
     }
-    if ((st = upload(e, e->dnodes, nodes.data(), nodes.size() * sizeof(NodeInfo)))) return st;
-    if ((st = upload(e, e->dchildren, children.data(), children.size() * sizeof(int32_t)))) return st;
     if ((st = upload(e, e->dleafof, leaf_of.data(), leaf_of.size() * sizeof(int32_t)))) return st;
     for (int i = 0; i < nitems; ++i) {
         ItemHost& it = e->items[i];
         const DatasetHost& d = e->ds[it.dataset];
-        Item& x = ditems[i];
-        x.nodes = e->dnodes.as<NodeInfo>() + it.nodes_off;
-        x.children = e->dchildren.as<int32_t>() + it.children_off;
-        x.codes = e->dcodes.as<uint8_t>() + d.codes_off;
-        x.weights = e->dweights.as<double>() + d.weights_off;
-        x.pi = e->dmodel.as<double>() + it.model_off + 2 * it.S * it.S + it.S + it.R;
-        x.P = e->dP.as<double>() + it.P_off;
-        x.dP = e->ddP.as<double>() + it.P_off;
-        x.PT = e->dPT.as<double>() + it.PT_off;
-        x.dPT = e->ddPT.as<double>() + it.PT_off;
-        x.F = e->dF.as<double>() + it.F_off;
-        x.G0 = e->dG0.as<double>() + it.G_off[0];
-        x.G1 = e->dG1.as<double>() + it.G_off[1];
         ModelDev& m = dmodels[i];
         const double* mb = e->dmodel.as<double>() + it.model_off;
         m.U = mb;
@@ -566,15 +545,83 @@ static int rebuild_schedule(mcp_engine* e) {
         m.PT = e->dPT.as<double>() + it.PT_off;
         m.dPT = e->ddPT.as<double>() + it.PT_off;
     }
-    if ((st = upload(e, e->ditems, ditems.data(), ditems.size() * sizeof(Item)))) return st;
     if ((st = upload(e, e->dmodels, dmodels.data(), dmodels.size() * sizeof(ModelDev)))) return st;
     if ((st = upload(e, e->delemP, elemP.data(), elemP.size() * sizeof(int32_t)))) return st;
     if ((st = upload(e, e->delemT, elemT.data(), elemT.size() * sizeof(int32_t)))) return st;
     e->nelemP = elemP[nitems];
     e->nelemT = elemT[nitems];
 
+    // children beyond the inline ones: reserve the overflow arena first so
+    // job records can point into it
+    size_t nmore = 0;
+    for (auto& it : e->items)
+        for (const NodeInfo& n : it.nodes) nmore += std::max(0, n.nchild - kInlineChildren);
+    if ((st = e->dmore.ensure(std::max<size_t>(nmore, 1) * sizeof(ChildRef)))) return st;
+    std::vector<ChildRef> more;
+    more.reserve(nmore);
+
+    auto child_ref = [&](const ItemHost& it, const DatasetHost& d, int u) {
+        const NodeInfo& n = it.nodes[u];
+        ChildRef c{};
+        const size_t slab = (size_t)it.R * d.Cp * it.S;
+        if (n.nchild == 0) {
+            c.F = nullptr;
+            c.codes = e->dcodes.as<uint8_t>() + d.codes_off + (size_t)n.leaf * d.Cp;
+            c.P = e->dPT.as<double>() + it.PT_off + (size_t)n.leaf * it.R * d.K * it.S;
+            c.dP = e->ddPT.as<double>() + it.PT_off + (size_t)n.leaf * it.R * d.K * it.S;
+            c.G = nullptr;
+        } else {
+            c.F = e->dF.as<double>() + it.F_off + (size_t)n.fslot * slab;
+            c.codes = nullptr;
+            c.P = e->dP.as<double>() + it.P_off + (size_t)u * it.R * it.S * it.S;
+            c.dP = e->ddP.as<double>() + it.P_off + (size_t)u * it.R * it.S * it.S;
+            double* gb = n.gbuf ? e->dG1.as<double>() + it.G_off[1] : e->dG0.as<double>() + it.G_off[0];
+            c.G = gb + (size_t)n.gslot * slab;
+        }
+        c.gpart = it.parts_off + n.gpart;
+        c.pad_ = 0;
+        return c;
+    };
+    auto make_job = [&](int i, int v) {
+        const ItemHost& it = e->items[i];
+        const DatasetHost& d = e->ds[it.dataset];
+        const NodeInfo& n = it.nodes[v];
+        const size_t slab = (size_t)it.R * d.Cp * it.S;
+        LevelJob J{};
+        J.S = it.S;
+        J.C = d.C;
+        J.Cp = d.Cp;
+        J.R = it.R;
+        J.K = d.K;
+        J.ntiles = it.ntiles;
+        J.nchild = n.nchild;
+        J.lpart = it.parts_off + n.lpart;
+        J.is_root = n.is_root;
+        J.F = n.is_root ? nullptr : e->dF.as<double>() + it.F_off + (size_t)n.fslot * slab;
+        if (n.is_root) {
+            J.G = nullptr;
+        } else {
+            const double* gb = n.gbuf ? e->dG1.as<double>() + it.G_off[1]
+                                      : e->dG0.as<double>() + it.G_off[0];
+            J.G = gb + (size_t)n.gslot * slab;
+        }
+        J.weights = e->dweights.as<double>() + d.weights_off;
+        J.pi = e->dmodel.as<double>() + it.model_off + 2 * it.S * it.S + it.S + it.R;
+        J.more = nullptr;
+        for (int q = 0; q < n.nchild; ++q) {
+            const ChildRef c = child_ref(it, d, it.children[n.first_child + q]);
+            if (q < kInlineChildren) {
+                J.kid[q] = c;
+            } else {
+                if (q == kInlineChildren) J.more = e->dmore.as<ChildRef>() + more.size();
+                more.push_back(c);
+            }
+        }
+        return J;
+    };
+
     // level schedule: post-order by height, pre-order by depth, per state count
-    e->jobs.clear();
+    std::vector<LevelJob> jf, ja;
     e->taskoff.clear();
     e->fwd.clear();
     e->adj.clear();
@@ -586,39 +633,52 @@ static int rebuild_schedule(mcp_engine* e) {
         if (std::find(svals.begin(), svals.end(), it.S) == svals.end()) svals.push_back(it.S);
     }
     std::sort(svals.begin(), svals.end());
-    auto add_launch = [&](bool adjoint, int S, int level) {
+    // node lists per (level) to avoid rescanning every item per level
+    std::vector<std::vector<std::pair<int, int>>> by_h(maxh + 1), by_d(maxd + 1);
+    for (int i = 0; i < nitems; ++i) {
+        const ItemHost& it = e->items[i];
+        for (int v = 0; v < it.N; ++v) {
+            if (it.nodes[v].nchild == 0) continue;
+            by_h[it.height[v]].push_back({i, v});
+            by_d[it.depth[v]].push_back({i, v});
+        }
+    }
+    auto add_launch = [&](bool adjoint, int S, const std::vector<std::pair<int, int>>& nodes) -> int {
+        std::vector<LevelJob>& js = adjoint ? ja : jf;
         Launch L;
         L.adjoint = adjoint;
         L.S = S;
-        L.job_begin = (int)e->jobs.size();
+        L.job_begin = (int)js.size();
         L.taskoff_begin = (int)e->taskoff.size();
-        int tasks = 0;
+        long long tasks = 0;
         e->taskoff.push_back(0);
-        for (int i = 0; i < nitems; ++i) {
+        for (auto [i, v] : nodes) {
             const ItemHost& it = e->items[i];
             if (it.S != S) continue;
-            for (int v = 0; v < it.N; ++v) {
-                if (it.nodes[v].nchild == 0) continue;
-                const int lv = adjoint ? it.depth[v] : it.height[v];
-                if (lv != level) continue;
-                e->jobs.push_back(Job{i, v});
-                tasks += it.R * it.ntiles;
-                e->taskoff.push_back(tasks);
-            }
+            js.push_back(make_job(i, v));
+            tasks += (long long)it.R * it.ntiles;
+            e->taskoff.push_back((int32_t)tasks);
         }
-        L.njobs = (int)e->jobs.size() - L.job_begin;
-        L.ntasks = tasks;
+        L.njobs = (int)js.size() - L.job_begin;
+        L.ntasks = (int)tasks;
         if (L.njobs == 0) {
             e->taskoff.pop_back();
-            return;
+            return MCP_OK;
         }
+        if (tasks > 0x7fffffffLL) return fail(MCP_ERR_UNSUPPORTED, "too many tasks in one level");
         (adjoint ? e->adj : e->fwd).push_back(L);
+        return MCP_OK;
     };
     for (int h = 1; h <= maxh; ++h)
-        for (int S : svals) add_launch(false, S, h);
+        for (int S : svals)
+            if ((st = add_launch(false, S, by_h[h]))) return st;
     for (int dd = 0; dd <= maxd; ++dd)
-        for (int S : svals) add_launch(true, S, dd);
-    if ((st = upload(e, e->djobs, e->jobs.data(), e->jobs.size() * sizeof(Job)))) return st;
+        for (int S : svals)
+            if ((st = add_launch(true, S, by_d[dd]))) return st;
+    if ((st = upload(e, e->djobs_f, jf.data(), jf.size() * sizeof(LevelJob)))) return st;
+    if ((st = upload(e, e->djobs_a, ja.data(), ja.size() * sizeof(LevelJob)))) return st;
+    if (!more.empty() && (st = upload(e, e->dmore, more.data(), more.size() * sizeof(ChildRef))))
+        return st;
     if ((st = upload(e, e->dtaskoff, e->taskoff.data(), e->taskoff.size() * sizeof(int32_t))))
         return st;
 
@@ -651,9 +711,9 @@ static int engine_create(mcp_engine** out, const mcp_batch_desc* d) {
     *out = nullptr;
     if (d->n_datasets < 1 || d->n_items < 1 || !d->datasets || !d->items)
         return fail(MCP_ERR_INVALID, "batch needs at least one dataset and one item");
-    int tile = d->tile ? d->tile : 128;
-    if (tile != 64 && tile != 128 && tile != 256)
-        return fail(MCP_ERR_INVALID, "tile must be 64, 128 or 256");
+    const int tpb = d->block_threads ? d->block_threads : 128;
+    if (tpb != 64 && tpb != 128 && tpb != 256)
+        return fail(MCP_ERR_INVALID, "block_threads must be 64, 128 or 256");
     int ndev = 0;
     HIP_TRY(hipGetDeviceCount(&ndev));
     if (d->device < 0 || d->device >= ndev)
@@ -662,7 +722,7 @@ static int engine_create(mcp_engine** out, const mcp_batch_desc* d) {
     std::unique_ptr<mcp_engine> e(new (std::nothrow) mcp_engine());
     if (!e) return fail(MCP_ERR_NOMEM, "host allocation failed");
     e->device = d->device;
-    e->tile = tile;
+    e->tpb = tpb;
     hipDeviceProp_t prop;
     HIP_TRY(hipGetDeviceProperties(&prop, d->device));
     e->num_cu = prop.multiProcessorCount;
@@ -670,10 +730,10 @@ static int engine_create(mcp_engine** out, const mcp_batch_desc* d) {
     int st;
     e->ds.resize(d->n_datasets);
     for (int k = 0; k < d->n_datasets; ++k)
-        if ((st = build_dataset(d->datasets[k], tile, e->ds[k]))) return st;
+        if ((st = build_dataset(d->datasets[k], tpb, e->ds[k]))) return st;
     e->items.resize(d->n_items);
     for (int i = 0; i < d->n_items; ++i)
-        if ((st = item_from_desc(d->items[i], e->ds, tile, e->items[i]))) return st;
+        if ((st = item_from_desc(d->items[i], e->ds, tpb, e->items[i]))) return st;
 
     // fixed layout
     size_t codes = 0, tiptab = 0, weights = 0;
@@ -837,15 +897,15 @@ static int enqueue_eval(mcp_engine* e, bool grad) {
                                (int)e->items.size(), e->nelemT, st));
     const int maxgrid = e->num_cu * 8;
     for (const Launch& L : e->fwd) {
-        TIMED(2, launch_level(false, L.S, e->ditems.as<Item>(), e->djobs.as<Job>() + L.job_begin,
+        TIMED(2, launch_level(false, L.S, e->djobs_f.as<LevelJob>() + L.job_begin,
                               e->dtaskoff.as<int32_t>() + L.taskoff_begin, L.njobs,
-                              e->dparts.as<double>(), std::min(L.ntasks, maxgrid), e->tile, st));
+                              e->dparts.as<double>(), std::min(L.ntasks, maxgrid), e->tpb, st));
     }
     if (grad)
         for (const Launch& L : e->adj) {
-            TIMED(3, launch_level(true, L.S, e->ditems.as<Item>(), e->djobs.as<Job>() + L.job_begin,
+            TIMED(3, launch_level(true, L.S, e->djobs_a.as<LevelJob>() + L.job_begin,
                                   e->dtaskoff.as<int32_t>() + L.taskoff_begin, L.njobs,
-                                  e->dparts.as<double>(), std::min(L.ntasks, maxgrid), e->tile,
+                                  e->dparts.as<double>(), std::min(L.ntasks, maxgrid), e->tpb,
                                   st));
         }
     const int nops = grad ? (int)e->red.size() : e->n_ll_ops;
@@ -963,7 +1023,7 @@ int mcp_plan_create(mcp_plan** out, const mcp_plan_desc* d) {
     bd.n_items = 1;
     bd.items = &it;
     bd.device = d->device;
-    bd.tile = 0;
+    bd.block_threads = 0;
     return engine_create(out, &bd);
 }
 

@@ -9,53 +9,43 @@
 // HBM layout of a partial slab (one internal node of one item):
 //   slab[r][c][s]  (state fastest, then pattern, then category) -- the
 //   reference's own data[s, c, r, v] order (LikelihoodCalculator_Node.jl) --
-//   with the pattern axis padded to Cp = roundup(C, 64) so every category
-//   slab starts on a 64-pattern boundary and a wavefront's loads are one
-//   contiguous span.
+//   with the pattern axis padded to Cp, a multiple of the task width, so
+//   every task's columns are one contiguous span.
 #pragma once
 #include <stdint.h>
 
 namespace mcp {
 
-constexpr int kMaxStates = 64;
-constexpr int kPadPatterns = 64;
-
-// Per-node record of one item (index = num-1).
-struct NodeInfo {
-    int32_t first_child;   // offset into Item::children
-    int32_t nchild;
-    int32_t fslot;         // partial slab index (internal non-root), else -1
-    int32_t leaf;          // row of the dataset's tip codes (leaves), else -1
-    int32_t gslot;         // pre-order slab index in ring buffer gbuf (internal non-root), else -1
-    int32_t gbuf;          // 0/1: which pre-order ring buffer (depth parity)
-    int32_t lpart;         // offset of this node's log-scaler partial sums in parts[] (internal), else -1
-    int32_t gpart;         // offset of this branch's gradient partial sums in parts[] (non-root), else -1
-    int32_t is_root;
+// A child of a level job's node, with every address resolved on the host so
+// a task reaches its operands after one round of scalar loads.
+struct ChildRef {
+    const double* F;       // internal child: partial slab at r = 0 ([R][Cp][S]); leaf: nullptr
+    const uint8_t* codes;  // leaf child: its tip-code row ([Cp]); internal: nullptr
+    const double* P;       // internal: P of the child's branch at r = 0 ([R][S][S], P[s][s1]);
+                           // leaf: P . tipvec rows at r = 0 ([R][K][S])
+    const double* dP;      // same layouts for dP/dt (pre-order pass)
+    double* G;             // internal child: its pre-order slab at r = 0; leaf: nullptr
+    int32_t gpart;         // offset of the branch's gradient partial sums
     int32_t pad_;
 };
 
-// Per-item device descriptor.
-struct Item {
-    int32_t S, C, Cp, R, K, N, ntiles, tile;   // tile == threads per task (patterns per task)
-    const NodeInfo* nodes;     // [N]
-    const int32_t* children;   // child indices (num-1)
-    const uint8_t* codes;      // [n_leaves][Cp] tip codes
-    const double* weights;     // [Cp] (zero in the padding)
-    const double* pi;          // [S]
-    const double* P;           // [N-1][R][S][S] row-major P[s][s1]
-    const double* dP;          // [N-1][R][S][S]
-    const double* PT;          // [n_leaves][R][K][S]  P . tipvec_k  (leaf branches)
-    const double* dPT;         // [n_leaves][R][K][S]  dP . tipvec_k
-    double* F;                 // [n_fslots][R][Cp][S]
-    double* G0;                // pre-order ring buffer 0 [g0 slots][R][Cp][S]
-    double* G1;                // pre-order ring buffer 1
-};
+constexpr int kInlineChildren = 3;
 
-// Work unit of one level launch: one node of one item; it expands to
-// R * ntiles tasks (one per (category, pattern tile)).
-struct Job {
-    int32_t item;
-    int32_t node;              // num-1
+// One internal node of one item.  Post-order launches read F, lpart,
+// is_root; pre-order launches read G (the node's own pre-order slab) and the
+// children's G/gpart.  A job expands into R * ntiles tasks.
+struct LevelJob {
+    int32_t S, C, Cp, R, K, ntiles;
+    int32_t nchild;
+    int32_t lpart;          // offset of this node's log-scaler partial sums
+    int32_t is_root;
+    int32_t pad_;
+    double* F;              // output partial slab at r = 0 (non-root), else nullptr
+    const double* G;        // own pre-order slab at r = 0 (non-root), else nullptr (root: pi)
+    const double* weights;  // [Cp], zero in the padding
+    const double* pi;       // [S]
+    const ChildRef* more;   // children beyond kInlineChildren
+    ChildRef kid[kInlineChildren];
 };
 
 // Inputs of the transition-matrix builder, per item.
@@ -69,9 +59,9 @@ struct ModelDev {
     const int32_t* leaf_of;    // [N] leaf row or -1
     double mu;
     int32_t S, R, N, K;
-    double* P;
+    double* P;                 // [N-1][R][S][S]
     double* dP;
-    double* PT;
+    double* PT;                // [n_leaves][R][K][S]
     double* dPT;
 };
 

@@ -11,13 +11,17 @@
 //   VectorizedFunctions.jl:2-11,51-74,170-177)                -> k_adj_level
 //   sums over (pattern, category) of log scalers / gradients  -> k_reduce
 //
-// Work decomposition of the level kernels: a job is one node of one item; it
-// expands into R * ntiles tasks, one per (category r, tile of `tile`
-// consecutive patterns).  One thread owns one pattern column of one
-// category and keeps the S-state vectors of that column in registers; the
-// transition matrices of a task are block-uniform, so they are read through
-// scalar loads and enter the FMAs as SGPR operands.  Blocks are persistent:
-// each strides over the task list of the launch.
+// Work decomposition of the level kernels: a job is one internal node of one
+// item (a LevelJob record with every operand address resolved on the host);
+// it expands into R * ntiles tasks, one per (category r, tile of
+// blockDim * CPT consecutive patterns).  Thread t of a block owns columns
+// tile_base + t + k * blockDim (k < CPT), so every load and store
+// instruction of a wavefront covers one contiguous span of slab rows, and
+// the S-state vectors of a column stay in registers.  The transition
+// matrices of a task are block-uniform: they are read with scalar loads and
+// enter the FMAs as SGPR operands.  Each block of a launch takes one
+// contiguous chunk of the launch's task list, so it locates its first job
+// once and then walks the jobs in order.
 //
 // Determinism: per-task sums are reduced with a fixed wavefront shuffle tree
 // plus a fixed LDS order, written to one slot per task, and summed by
@@ -93,30 +97,6 @@ __device__ __forceinline__ int find_job(const int32_t* __restrict__ task_off, in
     return lo;
 }
 
-// Message a child u sends up its branch in column (c, r):
-// msg[s] = sum_s1 P_u[s][s1] f_u[s1]; for a leaf it is the precomputed row
-// PT[leaf][r][code].  `f` receives the child's partial (internal children).
-template <int S>
-__device__ __forceinline__ void child_message(const Item& it, const NodeInfo& nu, int u, int r,
-                                              int c, double (&msg)[S], double (&f)[S]) {
-    if (nu.leaf >= 0) {
-        const int code = it.codes[(size_t)nu.leaf * it.Cp + c];
-        const double* pt = it.PT + (((size_t)nu.leaf * it.R + r) * it.K + code) * S;
-#pragma unroll
-        for (int s = 0; s < S; ++s) { msg[s] = pt[s]; f[s] = 0.0; }
-    } else {
-        load_row<S>(it.F + (((size_t)nu.fslot * it.R + r) * it.Cp + c) * S, f);
-        const double* __restrict__ P = it.P + ((size_t)u * it.R + r) * (S * S);
-#pragma unroll
-        for (int s = 0; s < S; ++s) {
-            double a = 0.0;
-#pragma unroll
-            for (int s1 = 0; s1 < S; ++s1) a = fma(f[s1], P[s * S + s1], a);
-            msg[s] = a;
-        }
-    }
-}
-
 // ---------------------------------------------------------------------------
 // transition matrices: P = U diag(exp(mu t D rho)) Uinv and dP/dt
 // ---------------------------------------------------------------------------
@@ -190,206 +170,264 @@ __global__ void k_tip_tables(const ModelDev* __restrict__ models, const int32_t*
 }
 
 // ---------------------------------------------------------------------------
-// post-order level: partial of every internal node of the level
+// level kernels
 // ---------------------------------------------------------------------------
 
-template <int S>
-__device__ __forceinline__ void fwd_task(const Item& it, int v, int r, int tile, double* red,
-                                         double* __restrict__ parts) {
-    const NodeInfo nv = it.nodes[v];
-    const int c = tile * it.tile + (int)threadIdx.x;
-    const bool valid = c < it.C;
-    double contrib = 0.0;
-    if (valid) {
-        double acc[S];
+__device__ __forceinline__ const ChildRef& child_of(const LevelJob& J, int q) {
+    return q < kInlineChildren ? J.kid[q] : J.more[q - kInlineChildren];
+}
+
+// Message each of a thread's CPT columns receives from child `ch` in
+// category r: msg[s] = sum_s1 P[s][s1] f[s1] (internal child, f loaded into
+// `f`), or the leaf's precomputed row PT[r][code] (`f` untouched).
+template <int S, int CPT>
+__device__ __forceinline__ void child_fetch(const ChildRef& ch, const LevelJob& J, int r, int c0,
+                                            int bd, double (&f)[CPT][S], double (&m)[CPT][S]) {
+    if (ch.codes) {
+        const double* pt = ch.P + (size_t)r * J.K * S;
 #pragma unroll
-        for (int s = 0; s < S; ++s) acc[s] = 1.0;
-        for (int q = 0; q < nv.nchild; ++q) {
-            const int u = it.children[nv.first_child + q];
-            const NodeInfo nu = it.nodes[u];
-            double msg[S], f[S];
-            child_message<S>(it, nu, u, r, c, msg, f);
+        for (int k = 0; k < CPT; ++k) {
+            const int c = c0 + k * bd;
+            const int code = c < J.C ? (int)ch.codes[c] : 0;
+            const double* row = pt + code * S;
 #pragma unroll
-            for (int s = 0; s < S; ++s) acc[s] *= msg[s];
+            for (int s = 0; s < S; ++s) m[k][s] = row[s];
         }
-        if (nv.is_root) {
+    } else {
+        const double* src = ch.F + (size_t)r * J.Cp * S;
+#pragma unroll
+        for (int k = 0; k < CPT; ++k) {
+            const int c = c0 + k * bd;
+            if (c < J.C) {
+                load_row<S>(src + (size_t)c * S, f[k]);
+            } else {
+#pragma unroll
+                for (int s = 0; s < S; ++s) f[k][s] = 1.0;
+            }
+        }
+        const double* __restrict__ P = ch.P + (size_t)r * (S * S);
+#pragma unroll
+        for (int k = 0; k < CPT; ++k)
+#pragma unroll
+            for (int s = 0; s < S; ++s) {
+                double a = 0.0;
+#pragma unroll
+                for (int s1 = 0; s1 < S; ++s1) a = fma(f[k][s1], P[s * S + s1], a);
+                m[k][s] = a;
+            }
+    }
+}
+
+// post-order: partial of the job's node for the thread's columns
+template <int S, int CPT>
+__device__ __forceinline__ void fwd_task(const LevelJob& J, int r, int tile, double* red,
+                                         double* __restrict__ parts) {
+    const int bd = blockDim.x;
+    const int c0 = tile * bd * CPT + (int)threadIdx.x;
+    double acc[CPT][S];
+#pragma unroll
+    for (int k = 0; k < CPT; ++k)
+#pragma unroll
+        for (int s = 0; s < S; ++s) acc[k][s] = 1.0;
+    for (int q = 0; q < J.nchild; ++q) {
+        double f[CPT][S], m[CPT][S];
+        child_fetch<S, CPT>(child_of(J, q), J, r, c0, bd, f, m);
+#pragma unroll
+        for (int k = 0; k < CPT; ++k)
+#pragma unroll
+            for (int s = 0; s < S; ++s) acc[k][s] *= m[k][s];
+    }
+    double contrib = 0.0;
+#pragma unroll
+    for (int k = 0; k < CPT; ++k) {
+        const int c = c0 + k * bd;
+        if (c >= J.C) continue;
+        if (J.is_root) {
             // root_sum: log sum_s pi_s Ftilde_s (VectorizedFunctions.jl:76-87)
             double t = 0.0;
 #pragma unroll
-            for (int s = 0; s < S; ++s) t = fma(acc[s], it.pi[s], t);
-            contrib = it.weights[c] * log(t);
+            for (int s = 0; s < S; ++s) t = fma(acc[k][s], J.pi[s], t);
+            contrib += J.weights[c] * log(t);
         } else {
             // by_max!: rescale by the state maximum, keep its log (VectorizedFunctions.jl:31-49)
-            double m = -INFINITY;
+            double mx = -INFINITY;
 #pragma unroll
-            for (int s = 0; s < S; ++s) m = (m < acc[s]) ? acc[s] : m;
-            const double inv = 1.0 / m;
+            for (int s = 0; s < S; ++s) mx = (mx < acc[k][s]) ? acc[k][s] : mx;
+            const double inv = 1.0 / mx;
 #pragma unroll
-            for (int s = 0; s < S; ++s) acc[s] *= inv;
-            store_row<S>(it.F + (((size_t)nv.fslot * it.R + r) * it.Cp + c) * S, acc);
-            contrib = it.weights[c] * log(m);
+            for (int s = 0; s < S; ++s) acc[k][s] *= inv;
+            store_row<S>(J.F + ((size_t)r * J.Cp + c) * S, acc[k]);
+            contrib += J.weights[c] * log(mx);
         }
     }
     const double bs = block_sum(contrib, red);
-    if (threadIdx.x == 0) parts[nv.lpart + r * it.ntiles + tile] = bs;
+    if (threadIdx.x == 0) parts[J.lpart + r * J.ntiles + tile] = bs;
 }
 
-template <int S>
-__global__ void __launch_bounds__(256) k_fwd_level(const Item* __restrict__ items,
-                                                   const Job* __restrict__ jobs,
-                                                   const int32_t* __restrict__ task_off, int njobs,
-                                                   double* __restrict__ parts) {
-    __shared__ double red[16];
-    const int total = task_off[njobs];
-    for (int t = blockIdx.x; t < total; t += gridDim.x) {
-        const int j = find_job(task_off, njobs, t);
-        const Job jb = jobs[j];
-        const Item& it = items[jb.item];
-        const int local = t - task_off[j];
-        const int r = local / it.ntiles;
-        const int tile = local - r * it.ntiles;
-        fwd_task<S>(it, jb.node, r, tile, red, parts);
-    }
-}
-
-// ---------------------------------------------------------------------------
-// pre-order level: for every internal parent p of the level and each child u
-//   T_u = G_p (.) prod_{w != u} msg_w                      (:28-37)
+// pre-order, one child u of the job's node p, given T_u = G_p (.) prod of
+// the siblings' messages:
 //   g_u = sum_m T_u[m] (dP_u f_u)[m]                        (:39-48)
 //   G_u[k] = sum_m P_u[m][k] T_u[m]                         (:50-57)
 //   col = sum_k f_u[k] G_u[k];  grad_u += w_c g_u / col     (:59-69)
 //   G_u /= max_k G_u[k] (internal u)                        (:71-73)
-// (line numbers: LikelihoodCalculator_Node.jl).  For a leaf u, g_u and col
-// are dot products of T_u with the leaf's dP.tip and P.tip rows.
-// ---------------------------------------------------------------------------
-
-template <int S>
-__device__ __forceinline__ double child_adjoint(const Item& it, const NodeInfo& nu, int u, int r,
-                                                int c, const double (&T)[S], const double (&f)[S],
-                                                const double (&msg)[S]) {
-    double num = 0.0, col = 0.0;
-    if (nu.leaf >= 0) {
-        const int code = it.codes[(size_t)nu.leaf * it.Cp + c];
-        const double* dpt = it.dPT + (((size_t)nu.leaf * it.R + r) * it.K + code) * S;
+// (LikelihoodCalculator_Node.jl).  For a leaf u, g_u and col are dot
+// products of T_u with the leaf's dP.tip and P.tip rows.
+template <int S, int CPT>
+__device__ __forceinline__ double child_adjoint(const ChildRef& ch, const LevelJob& J, int r, int c0,
+                                                int bd, const double (&T)[CPT][S],
+                                                const double (&f)[CPT][S],
+                                                const double (&m)[CPT][S]) {
+    double contrib = 0.0;
+    if (ch.codes) {
+        const double* dpt = ch.dP + (size_t)r * J.K * S;
 #pragma unroll
-        for (int m = 0; m < S; ++m) {
-            num = fma(T[m], dpt[m], num);
-            col = fma(T[m], msg[m], col);
-        }
-    } else {
-        const double* __restrict__ P = it.P + ((size_t)u * it.R + r) * (S * S);
-        const double* __restrict__ dP = it.dP + ((size_t)u * it.R + r) * (S * S);
-        double g[S];
+        for (int k = 0; k < CPT; ++k) {
+            const int c = c0 + k * bd;
+            if (c >= J.C) continue;
+            const double* row = dpt + (int)ch.codes[c] * S;
+            double num = 0.0, col = 0.0;
 #pragma unroll
-        for (int k = 0; k < S; ++k) g[k] = 0.0;
-#pragma unroll
-        for (int m = 0; m < S; ++m) {
-            double dm = 0.0;
-#pragma unroll
-            for (int k = 0; k < S; ++k) {
-                dm = fma(dP[m * S + k], f[k], dm);
-                g[k] = fma(P[m * S + k], T[m], g[k]);
+            for (int s = 0; s < S; ++s) {
+                num = fma(T[k][s], row[s], num);
+                col = fma(T[k][s], m[k][s], col);
             }
-            num = fma(T[m], dm, num);
+            contrib += J.weights[c] * (num / col);
         }
-        double mx = -INFINITY;
-#pragma unroll
-        for (int k = 0; k < S; ++k) {
-            col = fma(f[k], g[k], col);
-            mx = (mx < g[k]) ? g[k] : mx;
-        }
-        const double inv = 1.0 / mx;
-#pragma unroll
-        for (int k = 0; k < S; ++k) g[k] *= inv;
-        double* G = nu.gbuf ? it.G1 : it.G0;
-        store_row<S>(G + (((size_t)nu.gslot * it.R + r) * it.Cp + c) * S, g);
-    }
-    return it.weights[c] * (num / col);
-}
-
-template <int S>
-__device__ __forceinline__ void adj_task(const Item& it, int p, int r, int tile, double* red,
-                                         double* __restrict__ parts) {
-    const NodeInfo np = it.nodes[p];
-    const int c = tile * it.tile + (int)threadIdx.x;
-    const bool valid = c < it.C;
-    const int cc = valid ? c : 0;
-    double gp[S];
-    if (np.is_root) {
-#pragma unroll
-        for (int s = 0; s < S; ++s) gp[s] = it.pi[s];
     } else {
-        const double* G = np.gbuf ? it.G1 : it.G0;
-        if (valid) load_row<S>(G + (((size_t)np.gslot * it.R + r) * it.Cp + cc) * S, gp);
-        else {
+        const double* __restrict__ P = ch.P + (size_t)r * (S * S);
+        const double* __restrict__ dP = ch.dP + (size_t)r * (S * S);
+        double* Gdst = ch.G + (size_t)r * J.Cp * S;
 #pragma unroll
-            for (int s = 0; s < S; ++s) gp[s] = 0.0;
-        }
-    }
-    const int A = np.nchild;
-    if (A == 2) {
-        const int u0 = it.children[np.first_child];
-        const int u1 = it.children[np.first_child + 1];
-        const NodeInfo n0 = it.nodes[u0], n1 = it.nodes[u1];
-        double c0 = 0.0, c1 = 0.0;
-        if (valid) {
-            double m0[S], f0[S], m1[S], f1[S], T[S];
-            child_message<S>(it, n0, u0, r, cc, m0, f0);
-            child_message<S>(it, n1, u1, r, cc, m1, f1);
+        for (int k = 0; k < CPT; ++k) {
+            const int c = c0 + k * bd;
+            if (c >= J.C) continue;
+            double g[S];
 #pragma unroll
-            for (int s = 0; s < S; ++s) T[s] = gp[s] * m1[s];
-            c0 = child_adjoint<S>(it, n0, u0, r, cc, T, f0, m0);
+            for (int s = 0; s < S; ++s) g[s] = 0.0;
+            double num = 0.0;
 #pragma unroll
-            for (int s = 0; s < S; ++s) T[s] = gp[s] * m0[s];
-            c1 = child_adjoint<S>(it, n1, u1, r, cc, T, f1, m1);
-        }
-        const double s0 = block_sum(c0, red);
-        if (threadIdx.x == 0) parts[n0.gpart + r * it.ntiles + tile] = s0;
-        const double s1 = block_sum(c1, red);
-        if (threadIdx.x == 0) parts[n1.gpart + r * it.ntiles + tile] = s1;
-    } else {
-        // any other arity: recompute the siblings' messages per child
-        for (int q = 0; q < A; ++q) {
-            const int u = it.children[np.first_child + q];
-            const NodeInfo nu = it.nodes[u];
-            double cu = 0.0;
-            if (valid) {
-                double T[S], mu_[S], fu[S];
+            for (int mm = 0; mm < S; ++mm) {
+                double dm = 0.0;
 #pragma unroll
-                for (int s = 0; s < S; ++s) T[s] = gp[s];
-                for (int q2 = 0; q2 < A; ++q2) {
-                    if (q2 == q) continue;
-                    const int w = it.children[np.first_child + q2];
-                    const NodeInfo nw = it.nodes[w];
-                    double mw[S], fw[S];
-                    child_message<S>(it, nw, w, r, cc, mw, fw);
-#pragma unroll
-                    for (int s = 0; s < S; ++s) T[s] *= mw[s];
+                for (int s = 0; s < S; ++s) {
+                    dm = fma(dP[mm * S + s], f[k][s], dm);
+                    g[s] = fma(P[mm * S + s], T[k][mm], g[s]);
                 }
-                child_message<S>(it, nu, u, r, cc, mu_, fu);
-                cu = child_adjoint<S>(it, nu, u, r, cc, T, fu, mu_);
+                num = fma(T[k][mm], dm, num);
             }
-            const double su = block_sum(cu, red);
-            if (threadIdx.x == 0) parts[nu.gpart + r * it.ntiles + tile] = su;
+            double col = 0.0, mx = -INFINITY;
+#pragma unroll
+            for (int s = 0; s < S; ++s) {
+                col = fma(f[k][s], g[s], col);
+                mx = (mx < g[s]) ? g[s] : mx;
+            }
+            const double inv = 1.0 / mx;
+#pragma unroll
+            for (int s = 0; s < S; ++s) g[s] *= inv;
+            store_row<S>(Gdst + (size_t)c * S, g);
+            contrib += J.weights[c] * (num / col);
         }
+    }
+    return contrib;
+}
+
+template <int S, int CPT>
+__device__ __forceinline__ void adj_task(const LevelJob& J, int r, int tile, double* red,
+                                         double* __restrict__ parts) {
+    const int bd = blockDim.x;
+    const int c0 = tile * bd * CPT + (int)threadIdx.x;
+    double gp[CPT][S];
+    if (J.G) {
+        const double* src = J.G + (size_t)r * J.Cp * S;
+#pragma unroll
+        for (int k = 0; k < CPT; ++k) {
+            const int c = c0 + k * bd;
+            if (c < J.C) {
+                load_row<S>(src + (size_t)c * S, gp[k]);
+            } else {
+#pragma unroll
+                for (int s = 0; s < S; ++s) gp[k][s] = 1.0;
+            }
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < CPT; ++k)
+#pragma unroll
+            for (int s = 0; s < S; ++s) gp[k][s] = J.pi[s];
+    }
+    if (J.nchild == 2) {
+        const ChildRef& a = J.kid[0];
+        const ChildRef& b = J.kid[1];
+        double fa[CPT][S], ma[CPT][S], fb[CPT][S], mb[CPT][S], T[CPT][S];
+        child_fetch<S, CPT>(a, J, r, c0, bd, fa, ma);
+        child_fetch<S, CPT>(b, J, r, c0, bd, fb, mb);
+#pragma unroll
+        for (int k = 0; k < CPT; ++k)
+#pragma unroll
+            for (int s = 0; s < S; ++s) T[k][s] = gp[k][s] * mb[k][s];
+        const double ca = child_adjoint<S, CPT>(a, J, r, c0, bd, T, fa, ma);
+#pragma unroll
+        for (int k = 0; k < CPT; ++k)
+#pragma unroll
+            for (int s = 0; s < S; ++s) T[k][s] = gp[k][s] * ma[k][s];
+        const double cb = child_adjoint<S, CPT>(b, J, r, c0, bd, T, fb, mb);
+        const double sa = block_sum(ca, red);
+        if (threadIdx.x == 0) parts[a.gpart + r * J.ntiles + tile] = sa;
+        const double sb = block_sum(cb, red);
+        if (threadIdx.x == 0) parts[b.gpart + r * J.ntiles + tile] = sb;
+        return;
+    }
+    // any other arity: T_u from the siblings' messages, recomputed per child
+    for (int q = 0; q < J.nchild; ++q) {
+        const ChildRef& u = child_of(J, q);
+        double T[CPT][S], fu[CPT][S], mu_[CPT][S];
+#pragma unroll
+        for (int k = 0; k < CPT; ++k)
+#pragma unroll
+            for (int s = 0; s < S; ++s) T[k][s] = gp[k][s];
+        for (int q2 = 0; q2 < J.nchild; ++q2) {
+            if (q2 == q) continue;
+            double fw[CPT][S], mw[CPT][S];
+            child_fetch<S, CPT>(child_of(J, q2), J, r, c0, bd, fw, mw);
+#pragma unroll
+            for (int k = 0; k < CPT; ++k)
+#pragma unroll
+                for (int s = 0; s < S; ++s) T[k][s] *= mw[k][s];
+        }
+        child_fetch<S, CPT>(u, J, r, c0, bd, fu, mu_);
+        const double cu = child_adjoint<S, CPT>(u, J, r, c0, bd, T, fu, mu_);
+        const double su = block_sum(cu, red);
+        if (threadIdx.x == 0) parts[u.gpart + r * J.ntiles + tile] = su;
     }
 }
 
-template <int S>
-__global__ void __launch_bounds__(256) k_adj_level(const Item* __restrict__ items,
-                                                   const Job* __restrict__ jobs,
-                                                   const int32_t* __restrict__ task_off, int njobs,
-                                                   double* __restrict__ parts) {
+// One launch = one level (post-order: nodes of one height; pre-order: nodes
+// of one depth) of every item with S states.  Block b takes the contiguous
+// task chunk [b*chunk, (b+1)*chunk).
+template <int S, int CPT, bool ADJ>
+__global__ void __launch_bounds__(256) k_level(const LevelJob* __restrict__ jobs,
+                                               const int32_t* __restrict__ task_off, int njobs,
+                                               double* __restrict__ parts) {
     __shared__ double red[16];
     const int total = task_off[njobs];
-    for (int t = blockIdx.x; t < total; t += gridDim.x) {
-        const int j = find_job(task_off, njobs, t);
-        const Job jb = jobs[j];
-        const Item& it = items[jb.item];
-        const int local = t - task_off[j];
-        const int r = local / it.ntiles;
-        const int tile = local - r * it.ntiles;
-        adj_task<S>(it, jb.node, r, tile, red, parts);
+    const int chunk = (total + gridDim.x - 1) / gridDim.x;
+    const int t0 = blockIdx.x * chunk;
+    const int t1 = min(total, t0 + chunk);
+    if (t0 >= t1) return;
+    int j = find_job(task_off, njobs, t0);
+    int jbeg = task_off[j], jend = task_off[j + 1];
+    for (int t = t0; t < t1; ++t) {
+        while (t >= jend) {
+            ++j;
+            jbeg = jend;
+            jend = task_off[j + 1];
+        }
+        const LevelJob& J = jobs[j];
+        const int local = t - jbeg;
+        const int r = local / J.ntiles;
+        const int tile = local - r * J.ntiles;
+        if constexpr (ADJ) adj_task<S, CPT>(J, r, tile, red, parts);
+        else fwd_task<S, CPT>(J, r, tile, red, parts);
     }
 }
 
@@ -445,30 +483,39 @@ hipError_t launch_tip_tables(const ModelDev* models, const int32_t* elem_off, in
     return hipGetLastError();
 }
 
-#define MCP_FOR_EACH_S(X) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(16) X(20)
+// State counts with built kernels and the columns per thread used for each.
+#define MCP_FOR_EACH_S(X) X(1, 2) X(2, 4) X(3, 2) X(4, 2) X(5, 1) X(6, 1) X(7, 1) X(8, 1) X(16, 1) X(20, 1)
 
 bool supported_states(int S) {
     switch (S) {
-#define X(n) case n: return true;
+#define X(n, c) case n: return true;
         MCP_FOR_EACH_S(X)
 #undef X
         default: return false;
     }
 }
 
-hipError_t launch_level(bool adjoint, int S, const Item* items, const Job* jobs,
-                        const int32_t* task_off, int njobs, double* parts, int grid, int tpb,
-                        hipStream_t st) {
+int cols_per_thread(int S) {
+    switch (S) {
+#define X(n, c) case n: return c;
+        MCP_FOR_EACH_S(X)
+#undef X
+        default: return 1;
+    }
+}
+
+hipError_t launch_level(bool adjoint, int S, const LevelJob* jobs, const int32_t* task_off,
+                        int njobs, double* parts, int grid, int tpb, hipStream_t st) {
     if (njobs <= 0 || grid <= 0) return hipSuccess;
     switch (S) {
-#define X(n)                                                                                    \
-    case n:                                                                                     \
-        if (adjoint)                                                                            \
-            hipLaunchKernelGGL(k_adj_level<n>, dim3(grid), dim3(tpb), 0, st, items, jobs,       \
-                               task_off, njobs, parts);                                         \
-        else                                                                                    \
-            hipLaunchKernelGGL(k_fwd_level<n>, dim3(grid), dim3(tpb), 0, st, items, jobs,       \
-                               task_off, njobs, parts);                                         \
+#define X(n, c)                                                                                  \
+    case n:                                                                                      \
+        if (adjoint)                                                                             \
+            hipLaunchKernelGGL((k_level<n, c, true>), dim3(grid), dim3(tpb), 0, st, jobs,        \
+                               task_off, njobs, parts);                                          \
+        else                                                                                     \
+            hipLaunchKernelGGL((k_level<n, c, false>), dim3(grid), dim3(tpb), 0, st, jobs,       \
+                               task_off, njobs, parts);                                          \
         break;
         MCP_FOR_EACH_S(X)
 #undef X

@@ -166,7 +166,7 @@ class Batch(_Handle):
 
     _destroy = "mcp_batch_destroy"
 
-    def __init__(self, datasets, items, device: int = 0, tile: int = 0):
+    def __init__(self, datasets, items, device: int = 0, block_threads: int = 0):
         super().__init__()
         self._keep = []
         dsd = (_lib.DatasetDesc * len(datasets))()
@@ -205,7 +205,7 @@ class Batch(_Handle):
             self.item_nodes.append(p.shape[0])
             self.item_S.append(self.dataset_S[int(it["dataset"])])
             self.item_R.append(int(it["n_rates"]))
-        bd = _lib.BatchDesc(len(datasets), dsd, len(items), itd, int(device), int(tile))
+        bd = _lib.BatchDesc(len(datasets), dsd, len(items), itd, int(device), int(block_threads))
         check(_lib.load_library().mcp_batch_create(ctypes.byref(self._h), ctypes.byref(bd)))
         self._keep = []
         self.n_items = len(items)
