@@ -65,6 +65,7 @@ def bench_batched(data, steps, warmup, sample_trees):
     plain single-tree launch sequence on a sample of the same trees."""
     datasets, items, models, blv = data
     b = mcp.Batch(datasets, items)
+    b.set_schedule("fused")
     for i, m in enumerate(models):
         b.set_model(i, mcp.Model(*m))
     n_items, total_patterns = b.sizes()
@@ -79,16 +80,18 @@ def bench_batched(data, steps, warmup, sample_trees):
     ms, ph = b.launch_times()
     b.set_timing(False)
     bytes_fwd, bytes_adj = b.algorithmic_bytes()
-    fwd_ms, adj_ms = float(np.sum(ms[ph == 2])), float(np.sum(ms[ph == 3]))
+    fused_ms = float(np.sum(ms[ph == 3]))
     b.close()
     out = {"items": n_items, "patterns_per_step": total_patterns,
+           "schedule": "fused",
            "ms_per_step": 1e3 * wall / steps,
            "value": total_patterns * steps / wall,
-           "post_order_GBs": bytes_fwd / (fwd_ms * 1e-3) / 1e9,
-           "pre_order_GBs": bytes_adj / (adj_ms * 1e-3) / 1e9,
-           "post_order_ms": fwd_ms, "pre_order_ms": adj_ms,
+           "fused_kernel_ms": fused_ms,
+           "fused_kernel_GBs": (bytes_fwd + bytes_adj) / (fused_ms * 1e-3) / 1e9,
+           "algorithmic_GB_per_step": (bytes_fwd + bytes_adj) / 1e9,
            "launches": int(ms.shape[0])}
-    # plain single-tree sequence on the first chain of the first trees
+    # plain single-tree sequence (one launch per tree level) on the first
+    # chain of the first trees
     pat, tsum = 0, 0.0
     offs = np.concatenate([[0], np.cumsum([it["parent"].shape[0] - 1 for it in items])])
     chains = len(items) // len(datasets)
@@ -97,6 +100,7 @@ def bench_batched(data, steps, warmup, sample_trees):
         d = datasets[t]
         L, C = d["tip_codes"].shape
         pb = mcp.Batch([d], [dict(items[i], dataset=0)])
+        pb.set_schedule("levels")
         pb.set_model(0, mcp.Model(*models[i]))
         bl = blv[offs[i]:offs[i + 1]]
         pb.logl_grad(bl)
@@ -171,6 +175,7 @@ def main():
     blv = mcp.get_branchlength_vector(w.tree)
     plan = mcp.Plan(w.ta.parent, w.x, len(w.rates), weights=w.weights)
     plan.set_model(mcp.Model(w.U, w.D, w.Uinv, w.mu, w.rates, w.pi))
+    plan.set_schedule("fused")
     for _ in range(args.warmup):
         plan.logl_grad(blv)
     log(f"warmup done ({args.warmup}); timing {args.steps} steps")
@@ -205,15 +210,37 @@ def main():
     log(f"graph: {1e3 * gwall / args.steps:.3f} ms/step")
 
     bytes_fwd, bytes_adj = plan.algorithmic_bytes()
-    fwd_ms = float(np.sum(phases[ph == 2])) / args.steps
-    adj_ms = float(np.sum(phases[ph == 3])) / args.steps
-    n_fwd = int(np.sum(ph == 2))
-    n_adj = int(np.sum(ph == 3))
-    if adj_ms >= fwd_ms:
-        kname, kbytes, kms, nl = "k_adj_level<4>", bytes_adj, adj_ms, n_adj
-    else:
-        kname, kbytes, kms, nl = "k_fwd_level<4>", bytes_fwd, fwd_ms, n_fwd
+    fused_ms = float(np.sum(phases[ph == 3])) / args.steps
+    kname, kbytes, kms, nl = "k_fused<4,2,true>", bytes_fwd + bytes_adj, fused_ms, int(np.sum(ph == 3))
     achieved = kbytes / (kms * 1e-3) / 1e9
+
+    # the same evaluation with one launch per tree level
+    plan.set_schedule("levels")
+    for _ in range(3):
+        plan.logl_grad(blv)
+    t2 = time.perf_counter()
+    for _ in range(args.steps):
+        plan.logl_grad(blv)
+    lwall = time.perf_counter() - t2
+    plan.set_timing(True)
+    plan.logl_grad(blv)
+    lms, lph = plan.launch_times()
+    plan.set_timing(False)
+    plan.use_graph(True)
+    for _ in range(3):
+        plan.logl_grad(blv)
+    t3 = time.perf_counter()
+    for _ in range(args.steps):
+        plan.logl_grad(blv)
+    lgwall = time.perf_counter() - t3
+    plan.use_graph(False)
+    levels = {"value": w.C * args.steps / lwall, "graph_value": w.C * args.steps / lgwall,
+              "ms_per_step": 1e3 * lwall / args.steps,
+              "post_order_ms": float(np.sum(lms[lph == 2])),
+              "pre_order_ms": float(np.sum(lms[lph == 3])),
+              "launches": int(lms.shape[0])}
+    log(f"levels schedule: {1e3 * lwall / args.steps:.3f} ms/step eager, "
+        f"{1e3 * lgwall / args.steps:.3f} graph")
     result = {
         "metric": "logL+grad site-pattern evals/sec",
         "value": value,
@@ -236,11 +263,13 @@ def main():
                      "kernel": kname, "launches_per_step": nl,
                      "algorithmic_bytes_per_step": kbytes,
                      "kernel_ms_per_step": kms},
+        "schedule": "fused",
         "graph_value": gvalue,
         "phase_ms_per_step": {"P_build": float(np.sum(phases[ph == 1])) / args.steps,
-                              "post_order": fwd_ms, "pre_order": adj_ms,
+                              "fused_fwd_adj": fused_ms,
                               "reduce": float(np.sum(phases[ph == 4])) / args.steps},
         "algorithmic_bytes_per_step": {"post_order": bytes_fwd, "pre_order": bytes_adj},
+        "levels_schedule": levels,
     }
     if batch_data is not None:
         log("timing ragged batch")

@@ -120,8 +120,8 @@ typedef struct mcp_batch_desc {
     int32_t n_items;
     const mcp_item_desc* items;
     int32_t device;
-    int32_t block_threads;          /* threads per block of the level kernels: 64, 128
-                                       or 256; 0 = default (128) */
+    int32_t waves_per_block;        /* wavefronts per block of the fused schedule, 1..4;
+                                       0 = default (4) */
 } mcp_batch_desc;
 
 /* MultiplePhyloDist (Phylodist.jl:143-272) generalised to uneven trees,
@@ -147,10 +147,23 @@ int mcp_batch_logl_grad(mcp_batch* batch, const double* blv_concat, double* ll,
  * Applies to plans and batches (mcp_plan and mcp_batch are one type). */
 int mcp_use_graph(mcp_batch* batch, int32_t on);
 
+/* Launch schedule of an evaluation.
+ *  MCP_SCHEDULE_FUSED   one launch walks every level of a (tree, category,
+ *                       pattern tile) with block barriers between levels
+ *                       (default);
+ *  MCP_SCHEDULE_LEVELS  one launch per post-order level (nodes of equal
+ *                       height) and per pre-order level (nodes of equal
+ *                       depth), over every item. */
+#define MCP_SCHEDULE_FUSED 0
+#define MCP_SCHEDULE_LEVELS 1
+int mcp_set_schedule(mcp_batch* batch, int32_t mode);
+
 /* Time later evaluations with HIP events on the engine's stream (on != 0).
  * ms[0] whole sequence (first upload to last download); ms[1..4] summed
  * kernel durations of the P/dP build, post-order levels, pre-order levels
- * and reductions -- of the last evaluation. */
+ * and reductions -- of the last evaluation.  Under the fused schedule the
+ * single fused launch is counted as pre-order (phase 3) when the gradient is
+ * computed and as post-order (phase 2) otherwise. */
 int mcp_set_timing(mcp_batch* batch, int32_t on);
 int mcp_get_timing(mcp_batch* batch, double* ms5);
 

@@ -26,12 +26,15 @@ MCP_ERR_HIP = 5
 MCP_ERR_UNSUPPORTED = 6
 MCP_ERR_STATE = 7
 MCP_FLAG_COMPRESS_PATTERNS = 1
+MCP_SCHEDULE_FUSED = 0
+MCP_SCHEDULE_LEVELS = 1
 
 # Every symbol the header declares (checked by the CPU test suite).
 EXPORTED = (
     "mcp_plan_create", "mcp_plan_destroy", "mcp_set_model", "mcp_set_topology", "mcp_logl",
     "mcp_logl_grad", "mcp_batch_create", "mcp_batch_destroy", "mcp_batch_set_model",
     "mcp_batch_set_topology", "mcp_batch_logl", "mcp_batch_logl_grad", "mcp_use_graph",
+    "mcp_set_schedule",
     "mcp_set_timing", "mcp_get_timing", "mcp_get_launch_times", "mcp_get_bytes", "mcp_get_sizes",
     "mcp_last_error",
 )
@@ -85,7 +88,7 @@ class BatchDesc(ctypes.Structure):
     _fields_ = [
         ("n_datasets", ctypes.c_int32), ("datasets", ctypes.POINTER(DatasetDesc)),
         ("n_items", ctypes.c_int32), ("items", ctypes.POINTER(ItemDesc)),
-        ("device", ctypes.c_int32), ("block_threads", ctypes.c_int32),
+        ("device", ctypes.c_int32), ("waves_per_block", ctypes.c_int32),
     ]
 
 
@@ -126,6 +129,7 @@ def load_library():
         "mcp_batch_logl": (ctypes.c_int, [vp, _dp, _dp]),
         "mcp_batch_logl_grad": (ctypes.c_int, [vp, _dp, _dp, _dp]),
         "mcp_use_graph": (ctypes.c_int, [vp, ctypes.c_int32]),
+        "mcp_set_schedule": (ctypes.c_int, [vp, ctypes.c_int32]),
         "mcp_set_timing": (ctypes.c_int, [vp, ctypes.c_int32]),
         "mcp_get_timing": (ctypes.c_int, [vp, _dp]),
         "mcp_get_launch_times": (ctypes.c_int, [vp, _dp, _ip, ctypes.c_int32]),

@@ -26,8 +26,9 @@
 namespace mcp {
 hipError_t launch_build_P(const ModelDev*, const int32_t*, int, long long, hipStream_t);
 hipError_t launch_tip_tables(const ModelDev*, const int32_t*, int, long long, hipStream_t);
-hipError_t launch_level(bool, int, const LevelJob*, const int32_t*, int, double*, int, int,
-                        hipStream_t);
+hipError_t launch_level(bool, int, const LevelJob*, const int32_t*, int, double*, int, hipStream_t);
+hipError_t launch_fused(bool, int, const ItemSched*, int, int, int, const LevelJob*, const int32_t*,
+                        const LevelJob*, const int32_t*, double*, hipStream_t);
 int cols_per_thread(int S);
 hipError_t launch_reduce(const RedOp*, int, const double*, double*, hipStream_t);
 bool supported_states(int S);
@@ -176,18 +177,24 @@ struct Launch {
 
 struct mcp_engine {
     int device = 0;
-    int tpb = 128;  // threads per block of the level kernels
+    int waves = 4;     // wavefronts per block of the fused kernel
+    int schedule = MCP_SCHEDULE_FUSED;
     int num_cu = 256;
     hipStream_t stream = nullptr;
     std::vector<DatasetHost> ds;
     std::vector<ItemHost> items;
     // device arenas
     DevBuf dF, dG0, dG1, dP, ddP, dPT, ddPT, dparts, dout, dblv, dmodel, dcodes, dweights, dtiptab,
-        dleafof, dmodels, djobs_f, djobs_a, dmore, dtaskoff, dred, delemP, delemT;
+        dleafof, dmodels, djobs_f, djobs_a, dmore, dtaskoff, dred, delemP, delemT, dsched, dlvl_f,
+        dlvl_a;
     PinnedBuf hblv, hout;
     // schedule
     std::vector<Launch> fwd, adj;
     std::vector<int32_t> taskoff;
+    struct FusedLaunch {
+        int S, sched_begin, nitems, nblocks;
+    };
+    std::vector<FusedLaunch> fused;
     std::vector<RedOp> red;  // ll ops first (n_items), then gradient ops
     int n_ll_ops = 0;
     long long nelemP = 0, nelemT = 0;
@@ -240,7 +247,7 @@ struct DeviceGuard {
 // datasets: tip partials -> state codes (+ optional pattern compression)
 // ---------------------------------------------------------------------------
 
-int build_dataset(const mcp_dataset_desc& d, int tpb, DatasetHost& out) {
+int build_dataset(const mcp_dataset_desc& d, DatasetHost& out) {
     if (d.n_states < 1 || d.n_patterns < 1 || d.n_leaves < 1 || !d.leaf_nums ||
         (!d.tip_partials && (!d.tip_codes || !d.code_table || d.n_codes < 1)))
         return fail(MCP_ERR_INVALID, "dataset: sizes must be positive and pointers non-null");
@@ -330,7 +337,7 @@ int build_dataset(const mcp_dataset_desc& d, int tpb, DatasetHost& out) {
         w = wu;
     }
     out.C = Cu;
-    out.Cp = (int)round_up((size_t)Cu, (size_t)tpb * cols_per_thread(S));
+    out.Cp = (int)round_up((size_t)Cu, (size_t)64 * cols_per_thread(S));
     out.codes.assign((size_t)L * out.Cp, 0);
     for (int j = 0; j < L; ++j)
         for (int c = 0; c < Cu; ++c)
@@ -441,8 +448,7 @@ int build_topology(ItemHost& it, const DatasetHost& d, const int32_t* parent) {
     return MCP_OK;
 }
 
-int item_from_desc(const mcp_item_desc& d, const std::vector<DatasetHost>& ds, int tpb,
-                   ItemHost& it) {
+int item_from_desc(const mcp_item_desc& d, const std::vector<DatasetHost>& ds, ItemHost& it) {
     if (d.dataset < 0 || d.dataset >= (int)ds.size())
         return fail(MCP_ERR_INVALID, "item: dataset index out of range");
     if (!d.parent || d.n_nodes < 2 || d.n_rates < 1)
@@ -452,7 +458,7 @@ int item_from_desc(const mcp_item_desc& d, const std::vector<DatasetHost>& ds, i
     it.N = d.n_nodes;
     it.R = d.n_rates;
     it.S = D.S;
-    it.ntiles = D.Cp / (tpb * cols_per_thread(D.S));
+    it.ntiles = D.Cp / (64 * cols_per_thread(D.S));
     return build_topology(it, D, d.parent);
 }
 
@@ -625,6 +631,7 @@ static int rebuild_schedule(mcp_engine* e) {
     e->taskoff.clear();
     e->fwd.clear();
     e->adj.clear();
+    e->fused.clear();
     int maxh = 0, maxd = 0;
     std::vector<int> svals;
     for (auto& it : e->items) {
@@ -669,12 +676,69 @@ static int rebuild_schedule(mcp_engine* e) {
         (adjoint ? e->adj : e->fwd).push_back(L);
         return MCP_OK;
     };
-    for (int h = 1; h <= maxh; ++h)
-        for (int S : svals)
-            if ((st = add_launch(false, S, by_h[h]))) return st;
-    for (int dd = 0; dd <= maxd; ++dd)
-        for (int S : svals)
-            if ((st = add_launch(true, S, by_d[dd]))) return st;
+    if (e->schedule == MCP_SCHEDULE_LEVELS) {
+        for (int h = 1; h <= maxh; ++h)
+            for (int S : svals)
+                if ((st = add_launch(false, S, by_h[h]))) return st;
+        for (int dd = 0; dd <= maxd; ++dd)
+            for (int S : svals)
+                if ((st = add_launch(true, S, by_d[dd]))) return st;
+    } else {
+        // fused: jobs grouped by item, then level; per-item level offsets
+        std::vector<int32_t> lf, la;
+        std::vector<ItemSched> sched;
+        std::vector<std::vector<ItemSched>> per_s(svals.size());
+        for (int i = 0; i < nitems; ++i) {
+            const ItemHost& it = e->items[i];
+            const DatasetHost& d = e->ds[it.dataset];
+            (void)d;
+            ItemSched I{};
+            I.R = it.R;
+            I.ntiles = it.ntiles;
+            std::vector<std::vector<int>> hv(it.max_height + 1), dv(it.max_depth + 1);
+            for (int v = 0; v < it.N; ++v) {
+                if (it.nodes[v].nchild == 0) continue;
+                hv[it.height[v]].push_back(v);
+                dv[it.depth[v]].push_back(v);
+            }
+            I.lf0 = (int)lf.size();
+            for (int h = 1; h <= it.max_height; ++h) {
+                if (hv[h].empty()) continue;
+                lf.push_back((int32_t)jf.size());
+                for (int v : hv[h]) jf.push_back(make_job(i, v));
+            }
+            I.lf1 = (int)lf.size();
+            I.la0 = (int)la.size();
+            for (int dd = 0; dd <= it.max_depth; ++dd) {
+                if (dv[dd].empty()) continue;
+                la.push_back((int32_t)ja.size());
+                for (int v : dv[dd]) ja.push_back(make_job(i, v));
+            }
+            I.la1 = (int)la.size();
+            const size_t k = std::find(svals.begin(), svals.end(), it.S) - svals.begin();
+            per_s[k].push_back(I);
+        }
+        lf.push_back((int32_t)jf.size());
+        la.push_back((int32_t)ja.size());
+        for (size_t k = 0; k < svals.size(); ++k) {
+            mcp_engine::FusedLaunch F{};
+            F.S = svals[k];
+            F.sched_begin = (int)sched.size();
+            long long blocks = 0;
+            for (ItemSched I : per_s[k]) {
+                I.block_off = (int32_t)blocks;
+                blocks += (long long)I.R * I.ntiles;
+                sched.push_back(I);
+            }
+            if (blocks > 0x7fffffffLL) return fail(MCP_ERR_UNSUPPORTED, "too many tiles");
+            F.nitems = (int)per_s[k].size();
+            F.nblocks = (int)blocks;
+            if (F.nitems) e->fused.push_back(F);
+        }
+        if ((st = upload(e, e->dsched, sched.data(), sched.size() * sizeof(ItemSched)))) return st;
+        if ((st = upload(e, e->dlvl_f, lf.data(), lf.size() * sizeof(int32_t)))) return st;
+        if ((st = upload(e, e->dlvl_a, la.data(), la.size() * sizeof(int32_t)))) return st;
+    }
     if ((st = upload(e, e->djobs_f, jf.data(), jf.size() * sizeof(LevelJob)))) return st;
     if ((st = upload(e, e->djobs_a, ja.data(), ja.size() * sizeof(LevelJob)))) return st;
     if (!more.empty() && (st = upload(e, e->dmore, more.data(), more.size() * sizeof(ChildRef))))
@@ -711,9 +775,8 @@ static int engine_create(mcp_engine** out, const mcp_batch_desc* d) {
     *out = nullptr;
     if (d->n_datasets < 1 || d->n_items < 1 || !d->datasets || !d->items)
         return fail(MCP_ERR_INVALID, "batch needs at least one dataset and one item");
-    const int tpb = d->block_threads ? d->block_threads : 128;
-    if (tpb != 64 && tpb != 128 && tpb != 256)
-        return fail(MCP_ERR_INVALID, "block_threads must be 64, 128 or 256");
+    const int waves = d->waves_per_block ? d->waves_per_block : 4;
+    if (waves < 1 || waves > 4) return fail(MCP_ERR_INVALID, "waves_per_block must be 1..4");
     int ndev = 0;
     HIP_TRY(hipGetDeviceCount(&ndev));
     if (d->device < 0 || d->device >= ndev)
@@ -722,7 +785,7 @@ static int engine_create(mcp_engine** out, const mcp_batch_desc* d) {
     std::unique_ptr<mcp_engine> e(new (std::nothrow) mcp_engine());
     if (!e) return fail(MCP_ERR_NOMEM, "host allocation failed");
     e->device = d->device;
-    e->tpb = tpb;
+    e->waves = waves;
     hipDeviceProp_t prop;
     HIP_TRY(hipGetDeviceProperties(&prop, d->device));
     e->num_cu = prop.multiProcessorCount;
@@ -730,10 +793,10 @@ static int engine_create(mcp_engine** out, const mcp_batch_desc* d) {
     int st;
     e->ds.resize(d->n_datasets);
     for (int k = 0; k < d->n_datasets; ++k)
-        if ((st = build_dataset(d->datasets[k], tpb, e->ds[k]))) return st;
+        if ((st = build_dataset(d->datasets[k], e->ds[k]))) return st;
     e->items.resize(d->n_items);
     for (int i = 0; i < d->n_items; ++i)
-        if ((st = item_from_desc(d->items[i], e->ds, tpb, e->items[i]))) return st;
+        if ((st = item_from_desc(d->items[i], e->ds, e->items[i]))) return st;
 
     // fixed layout
     size_t codes = 0, tiptab = 0, weights = 0;
@@ -895,19 +958,25 @@ static int enqueue_eval(mcp_engine* e, bool grad) {
                             (int)e->items.size(), e->nelemP, st));
     TIMED(1, launch_tip_tables(e->dmodels.as<ModelDev>(), e->delemT.as<int32_t>(),
                                (int)e->items.size(), e->nelemT, st));
-    const int maxgrid = e->num_cu * 8;
+    const int maxgrid = e->num_cu * 32;
     for (const Launch& L : e->fwd) {
         TIMED(2, launch_level(false, L.S, e->djobs_f.as<LevelJob>() + L.job_begin,
                               e->dtaskoff.as<int32_t>() + L.taskoff_begin, L.njobs,
-                              e->dparts.as<double>(), std::min(L.ntasks, maxgrid), e->tpb, st));
+                              e->dparts.as<double>(), std::min(L.ntasks, maxgrid), st));
     }
     if (grad)
         for (const Launch& L : e->adj) {
             TIMED(3, launch_level(true, L.S, e->djobs_a.as<LevelJob>() + L.job_begin,
                                   e->dtaskoff.as<int32_t>() + L.taskoff_begin, L.njobs,
-                                  e->dparts.as<double>(), std::min(L.ntasks, maxgrid), e->tpb,
-                                  st));
+                                  e->dparts.as<double>(), std::min(L.ntasks, maxgrid), st));
         }
+    for (const auto& F : e->fused) {
+        TIMED(grad ? 3 : 2,
+              launch_fused(grad, F.S, e->dsched.as<ItemSched>() + F.sched_begin, F.nitems,
+                           F.nblocks, e->waves, e->djobs_f.as<LevelJob>(), e->dlvl_f.as<int32_t>(),
+                           e->djobs_a.as<LevelJob>(), e->dlvl_a.as<int32_t>(),
+                           e->dparts.as<double>(), st));
+    }
     const int nops = grad ? (int)e->red.size() : e->n_ll_ops;
     TIMED(4, launch_reduce(e->dred.as<RedOp>(), nops, e->dparts.as<double>(),
                            e->dout.as<double>(), st));
@@ -1023,7 +1092,7 @@ int mcp_plan_create(mcp_plan** out, const mcp_plan_desc* d) {
     bd.n_items = 1;
     bd.items = &it;
     bd.device = d->device;
-    bd.block_threads = 0;
+    bd.waves_per_block = 0;
     return engine_create(out, &bd);
 }
 
@@ -1076,6 +1145,17 @@ int mcp_use_graph(mcp_batch* b, int32_t on) {
     b->use_graph = on != 0;
     if (!b->use_graph) b->drop_graphs();
     return MCP_OK;
+}
+
+int mcp_set_schedule(mcp_batch* b, int32_t mode) {
+    if (!b) return fail(MCP_ERR_INVALID, "null handle");
+    if (mode != MCP_SCHEDULE_FUSED && mode != MCP_SCHEDULE_LEVELS)
+        return fail(MCP_ERR_INVALID, "unknown schedule");
+    DeviceGuard guard(b->device);
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    if (b->schedule == mode) return MCP_OK;
+    b->schedule = mode;
+    return rebuild_schedule(b);
 }
 
 int mcp_set_timing(mcp_batch* b, int32_t on) {

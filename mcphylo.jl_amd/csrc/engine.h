@@ -65,6 +65,15 @@ struct ModelDev {
     double* dPT;
 };
 
+// Per-item record of the fused schedule (one launch per state count).
+struct ItemSched {
+    int32_t block_off;     // first block of this item in the launch
+    int32_t R, ntiles;
+    int32_t lf0, lf1;      // post-order levels: [lf0, lf1) into lvl_f (lvl_f[k] = first job)
+    int32_t la0, la1;      // pre-order levels:  [la0, la1) into lvl_a
+    int32_t pad_;
+};
+
 // Reduction descriptor: out[dst] = sum(parts[start .. start+len)).
 struct RedOp {
     int32_t dst;

@@ -11,17 +11,21 @@
 //   VectorizedFunctions.jl:2-11,51-74,170-177)                -> k_adj_level
 //   sums over (pattern, category) of log scalers / gradients  -> k_reduce
 //
-// Work decomposition of the level kernels: a job is one internal node of one
-// item (a LevelJob record with every operand address resolved on the host);
-// it expands into R * ntiles tasks, one per (category r, tile of
-// blockDim * CPT consecutive patterns).  Thread t of a block owns columns
-// tile_base + t + k * blockDim (k < CPT), so every load and store
-// instruction of a wavefront covers one contiguous span of slab rows, and
-// the S-state vectors of a column stay in registers.  The transition
-// matrices of a task are block-uniform: they are read with scalar loads and
-// enter the FMAs as SGPR operands.  Each block of a launch takes one
-// contiguous chunk of the launch's task list, so it locates its first job
-// once and then walks the jobs in order.
+// Work decomposition: a job is one internal node of one item (a LevelJob
+// record with every operand address resolved on the host); it expands into
+// R * ntiles tasks, one per (category r, tile of 64 * CPT consecutive
+// patterns).  A task is executed by one wavefront: lane l owns columns
+// tile_base + l + 64 k (k < CPT), so every load and store instruction covers
+// one contiguous span of slab rows, and the S-state vectors of a column stay
+// in registers.  The transition matrices of a task are wave-uniform: they
+// are read with scalar loads and enter the FMAs as SGPR operands.
+// Two schedules run the same tasks:
+//   k_level  one launch per tree level (post-order by height, pre-order by
+//            depth); each single-wave block takes one contiguous chunk of
+//            the launch's task list, locates its first job once and then
+//            walks the jobs in order;
+//   k_fused  one launch per evaluation; a block owns one (item, category,
+//            tile) and walks every level of its item (see below).
 //
 // Determinism: per-task sums are reduced with a fixed wavefront shuffle tree
 // plus a fixed LDS order, written to one slot per task, and summed by
@@ -220,10 +224,11 @@ __device__ __forceinline__ void child_fetch(const ChildRef& ch, const LevelJob& 
 
 // post-order: partial of the job's node for the thread's columns
 template <int S, int CPT>
-__device__ __forceinline__ void fwd_task(const LevelJob& J, int r, int tile, double* red,
+__device__ __forceinline__ void fwd_task(const LevelJob& J, int r, int tile,
                                          double* __restrict__ parts) {
-    const int bd = blockDim.x;
-    const int c0 = tile * bd * CPT + (int)threadIdx.x;
+    constexpr int bd = 64;
+    const int lane = (int)(threadIdx.x & 63);
+    const int c0 = tile * bd * CPT + lane;
     double acc[CPT][S];
 #pragma unroll
     for (int k = 0; k < CPT; ++k)
@@ -260,8 +265,8 @@ __device__ __forceinline__ void fwd_task(const LevelJob& J, int r, int tile, dou
             contrib += J.weights[c] * log(mx);
         }
     }
-    const double bs = block_sum(contrib, red);
-    if (threadIdx.x == 0) parts[J.lpart + r * J.ntiles + tile] = bs;
+    const double bs = wave_sum(contrib);
+    if (lane == 0) parts[J.lpart + r * J.ntiles + tile] = bs;
 }
 
 // pre-order, one child u of the job's node p, given T_u = G_p (.) prod of
@@ -332,10 +337,11 @@ __device__ __forceinline__ double child_adjoint(const ChildRef& ch, const LevelJ
 }
 
 template <int S, int CPT>
-__device__ __forceinline__ void adj_task(const LevelJob& J, int r, int tile, double* red,
+__device__ __forceinline__ void adj_task(const LevelJob& J, int r, int tile,
                                          double* __restrict__ parts) {
-    const int bd = blockDim.x;
-    const int c0 = tile * bd * CPT + (int)threadIdx.x;
+    constexpr int bd = 64;
+    const int lane = (int)(threadIdx.x & 63);
+    const int c0 = tile * bd * CPT + lane;
     double gp[CPT][S];
     if (J.G) {
         const double* src = J.G + (size_t)r * J.Cp * S;
@@ -371,10 +377,12 @@ __device__ __forceinline__ void adj_task(const LevelJob& J, int r, int tile, dou
 #pragma unroll
             for (int s = 0; s < S; ++s) T[k][s] = gp[k][s] * ma[k][s];
         const double cb = child_adjoint<S, CPT>(b, J, r, c0, bd, T, fb, mb);
-        const double sa = block_sum(ca, red);
-        if (threadIdx.x == 0) parts[a.gpart + r * J.ntiles + tile] = sa;
-        const double sb = block_sum(cb, red);
-        if (threadIdx.x == 0) parts[b.gpart + r * J.ntiles + tile] = sb;
+        const double sa = wave_sum(ca);
+        const double sb = wave_sum(cb);
+        if (lane == 0) {
+            parts[a.gpart + r * J.ntiles + tile] = sa;
+            parts[b.gpart + r * J.ntiles + tile] = sb;
+        }
         return;
     }
     // any other arity: T_u from the siblings' messages, recomputed per child
@@ -396,19 +404,18 @@ __device__ __forceinline__ void adj_task(const LevelJob& J, int r, int tile, dou
         }
         child_fetch<S, CPT>(u, J, r, c0, bd, fu, mu_);
         const double cu = child_adjoint<S, CPT>(u, J, r, c0, bd, T, fu, mu_);
-        const double su = block_sum(cu, red);
-        if (threadIdx.x == 0) parts[u.gpart + r * J.ntiles + tile] = su;
+        const double su = wave_sum(cu);
+        if (lane == 0) parts[u.gpart + r * J.ntiles + tile] = su;
     }
 }
 
 // One launch = one level (post-order: nodes of one height; pre-order: nodes
-// of one depth) of every item with S states.  Block b takes the contiguous
-// task chunk [b*chunk, (b+1)*chunk).
+// of one depth) of every item with S states.  Blocks are single wavefronts;
+// block b takes the contiguous task chunk [b*chunk, (b+1)*chunk).
 template <int S, int CPT, bool ADJ>
-__global__ void __launch_bounds__(256) k_level(const LevelJob* __restrict__ jobs,
-                                               const int32_t* __restrict__ task_off, int njobs,
-                                               double* __restrict__ parts) {
-    __shared__ double red[16];
+__global__ void __launch_bounds__(64) k_level(const LevelJob* __restrict__ jobs,
+                                              const int32_t* __restrict__ task_off, int njobs,
+                                              double* __restrict__ parts) {
     const int total = task_off[njobs];
     const int chunk = (total + gridDim.x - 1) / gridDim.x;
     const int t0 = blockIdx.x * chunk;
@@ -426,8 +433,48 @@ __global__ void __launch_bounds__(256) k_level(const LevelJob* __restrict__ jobs
         const int local = t - jbeg;
         const int r = local / J.ntiles;
         const int tile = local - r * J.ntiles;
-        if constexpr (ADJ) adj_task<S, CPT>(J, r, tile, red, parts);
-        else fwd_task<S, CPT>(J, r, tile, red, parts);
+        if constexpr (ADJ) adj_task<S, CPT>(J, r, tile, parts);
+        else fwd_task<S, CPT>(J, r, tile, parts);
+    }
+}
+
+// Fused schedule: every (item, category, pattern tile) is independent of all
+// other tiles through the whole evaluation, so one block walks all
+// post-order levels of its tile and then all pre-order levels, with a
+// block barrier between levels.  Within a level the block's wavefronts take
+// the level's nodes round robin (one node per wavefront, the wavefront's
+// lanes covering the tile's columns).  A tile's partials are written and
+// read back by the same block, hence on the same XCD and its L2.
+template <int S, int CPT, bool GRAD>
+__global__ void __launch_bounds__(256) k_fused(const ItemSched* __restrict__ items, int nitems,
+                                               const LevelJob* __restrict__ jobs_f,
+                                               const int32_t* __restrict__ lvl_f,
+                                               const LevelJob* __restrict__ jobs_a,
+                                               const int32_t* __restrict__ lvl_a,
+                                               double* __restrict__ parts) {
+    int lo = 0, hi = nitems;
+    const int b = (int)blockIdx.x;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (items[mid].block_off <= b) lo = mid; else hi = mid;
+    }
+    const ItemSched I = items[lo];
+    const int local = b - I.block_off;
+    const int r = local / I.ntiles;
+    const int tile = local - r * I.ntiles;
+    const int w = (int)(threadIdx.x >> 6);
+    const int W = (int)(blockDim.x >> 6);
+    for (int k = I.lf0; k < I.lf1; ++k) {
+        const int j1 = lvl_f[k + 1];
+        for (int j = lvl_f[k] + w; j < j1; j += W) fwd_task<S, CPT>(jobs_f[j], r, tile, parts);
+        __syncthreads();
+    }
+    if constexpr (GRAD) {
+        for (int k = I.la0; k < I.la1; ++k) {
+            const int j1 = lvl_a[k + 1];
+            for (int j = lvl_a[k] + w; j < j1; j += W) adj_task<S, CPT>(jobs_a[j], r, tile, parts);
+            __syncthreads();
+        }
     }
 }
 
@@ -505,17 +552,39 @@ int cols_per_thread(int S) {
 }
 
 hipError_t launch_level(bool adjoint, int S, const LevelJob* jobs, const int32_t* task_off,
-                        int njobs, double* parts, int grid, int tpb, hipStream_t st) {
+                        int njobs, double* parts, int grid, hipStream_t st) {
     if (njobs <= 0 || grid <= 0) return hipSuccess;
     switch (S) {
 #define X(n, c)                                                                                  \
     case n:                                                                                      \
         if (adjoint)                                                                             \
-            hipLaunchKernelGGL((k_level<n, c, true>), dim3(grid), dim3(tpb), 0, st, jobs,        \
+            hipLaunchKernelGGL((k_level<n, c, true>), dim3(grid), dim3(64), 0, st, jobs,         \
                                task_off, njobs, parts);                                          \
         else                                                                                     \
-            hipLaunchKernelGGL((k_level<n, c, false>), dim3(grid), dim3(tpb), 0, st, jobs,       \
+            hipLaunchKernelGGL((k_level<n, c, false>), dim3(grid), dim3(64), 0, st, jobs,        \
                                task_off, njobs, parts);                                          \
+        break;
+        MCP_FOR_EACH_S(X)
+#undef X
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_fused(bool grad, int S, const ItemSched* items, int nitems, int nblocks,
+                        int waves, const LevelJob* jobs_f, const int32_t* lvl_f,
+                        const LevelJob* jobs_a, const int32_t* lvl_a, double* parts,
+                        hipStream_t st) {
+    if (nitems <= 0 || nblocks <= 0) return hipSuccess;
+    switch (S) {
+#define X(n, c)                                                                                  \
+    case n:                                                                                      \
+        if (grad)                                                                                \
+            hipLaunchKernelGGL((k_fused<n, c, true>), dim3(nblocks), dim3(64 * waves), 0, st,    \
+                               items, nitems, jobs_f, lvl_f, jobs_a, lvl_a, parts);              \
+        else                                                                                     \
+            hipLaunchKernelGGL((k_fused<n, c, false>), dim3(nblocks), dim3(64 * waves), 0, st,   \
+                               items, nitems, jobs_f, lvl_f, jobs_a, lvl_a, parts);              \
         break;
         MCP_FOR_EACH_S(X)
 #undef X

@@ -60,6 +60,11 @@ class _Handle:
     def use_graph(self, on: bool = True) -> None:
         check(_lib.load_library().mcp_use_graph(self._h, 1 if on else 0))
 
+    def set_schedule(self, mode: str) -> None:
+        """'fused' (one launch per evaluation) or 'levels' (one launch per tree level)."""
+        m = {"fused": _lib.MCP_SCHEDULE_FUSED, "levels": _lib.MCP_SCHEDULE_LEVELS}[mode]
+        check(_lib.load_library().mcp_set_schedule(self._h, m))
+
     def set_timing(self, on: bool = True) -> None:
         check(_lib.load_library().mcp_set_timing(self._h, 1 if on else 0))
 
@@ -166,7 +171,7 @@ class Batch(_Handle):
 
     _destroy = "mcp_batch_destroy"
 
-    def __init__(self, datasets, items, device: int = 0, block_threads: int = 0):
+    def __init__(self, datasets, items, device: int = 0, waves_per_block: int = 0):
         super().__init__()
         self._keep = []
         dsd = (_lib.DatasetDesc * len(datasets))()
@@ -205,7 +210,7 @@ class Batch(_Handle):
             self.item_nodes.append(p.shape[0])
             self.item_S.append(self.dataset_S[int(it["dataset"])])
             self.item_R.append(int(it["n_rates"]))
-        bd = _lib.BatchDesc(len(datasets), dsd, len(items), itd, int(device), int(block_threads))
+        bd = _lib.BatchDesc(len(datasets), dsd, len(items), itd, int(device), int(waves_per_block))
         check(_lib.load_library().mcp_batch_create(ctypes.byref(self._h), ctypes.byref(bd)))
         self._keep = []
         self.n_items = len(items)

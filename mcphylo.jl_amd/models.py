@@ -24,6 +24,12 @@ class ComplexEigensystemError(ValueError):
     such a P (`VectorizedFunctions.jl:155-168`), so this path rejects it."""
 
 
+def _recip(x) -> float:
+    """1/x with IEEE semantics (1/0 = Inf), as Julia computes it."""
+    with np.errstate(divide="ignore"):
+        return float(np.float64(1.0) / np.float64(x))
+
+
 def _eigen(Q: np.ndarray):
     D, U = np.linalg.eig(Q)
     if np.iscomplexobj(D):
@@ -47,7 +53,7 @@ def Restriction(base_freq, substitution_rates=None):
     D = np.array([-1.0, 0.0])
     U = np.array([[f[1], f[1]], [-1.0 + f[1], 0.0 + f[1]]], order="F")
     Uinv = np.asfortranarray(np.linalg.inv(U))
-    mu = 1.0 / (2.0 * float(np.prod(f)))
+    mu = _recip(2.0 * float(np.prod(f)))
     return U, D, Uinv, mu
 
 
@@ -60,7 +66,7 @@ def JC(base_freq, substitution_rates=None):
     Q = np.full((S, S), off)
     Q[np.diag_indices(S)] = -diag
     U, D, Uinv = _eigen(Q)
-    return U, D, Uinv, 1.0 / diag
+    return U, D, Uinv, _recip(diag)
 
 
 def setmatrix(vec_vals) -> np.ndarray:
@@ -93,7 +99,7 @@ def GTR(base_freq, substitution_rates):
     dia = Q.sum(axis=1)
     Q[np.diag_indices(S)] = -dia
     U, D, Uinv = _eigen(Q)
-    return U, D, Uinv, 1.0 / float(dia.sum())
+    return U, D, Uinv, _recip(dia.sum())
 
 
 def freeK(base_freq, substitution_rates):
@@ -111,7 +117,7 @@ def freeK(base_freq, substitution_rates):
     dia = Q.sum(axis=1)
     Q[np.diag_indices(S)] = -dia
     U, D, Uinv = _eigen(Q)
-    return U, D, Uinv, 1.0 / float(dia.sum())
+    return U, D, Uinv, _recip(dia.sum())
 
 
 def stationary_freeK(substitution_rates) -> np.ndarray:

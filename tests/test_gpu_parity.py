@@ -33,10 +33,14 @@ def _oracle(w, x=None, weights=None, blv=None, grad=True):
                             blv, weights=weights if weights is not None else w.weights, grad=grad)
 
 
-def _plan(w, **kw):
+def _plan(w, schedule="fused", **kw):
     p = mcp.Plan(w.ta.parent, w.x, len(w.rates), weights=w.weights, **kw)
     p.set_model(mcp.Model(w.U, w.D, w.Uinv, w.mu, w.rates, w.pi))
+    p.set_schedule(schedule)
     return p
+
+
+SCHEDULES = ["fused", "levels"]
 
 
 def test_primates_golden_phylodist():
@@ -78,9 +82,10 @@ CASES = [
 ]
 
 
+@pytest.mark.parametrize("schedule", SCHEDULES)
 @pytest.mark.parametrize("seed,n,S,C,R,topo,gaps", CASES)
-def test_random_cases_vs_oracle(seed, n, S, C, R, topo, gaps):
-    print(f"case seed={seed} n={n} S={S} C={C} R={R} {topo}", flush=True)
+def test_random_cases_vs_oracle(seed, n, S, C, R, topo, gaps, schedule):
+    print(f"case seed={seed} n={n} S={S} C={C} R={R} {topo} {schedule}", flush=True)
     w = mcp.synthetic.make_workload("custom", seed=seed, n=n, S=S, C=C, R=R, topology=topo)
     x = np.array(w.x, order="F")
     if gaps:
@@ -90,21 +95,22 @@ def test_random_cases_vs_oracle(seed, n, S, C, R, topo, gaps):
             x[:, cols, v - 1] = 1.0
     w.x = x
     llo, go = _oracle(w)
-    p = _plan(w)
+    p = _plan(w, schedule)
     ll, g = p.logl_grad(mcp.get_branchlength_vector(w.tree))
     assert rel_err(ll, llo) < LL_TOL
     assert grad_err(g, go) < GRAD_TOL
     assert rel_err(p.logl(mcp.get_branchlength_vector(w.tree)), llo) < LL_TOL
 
 
-def test_gtr100_full_size_vs_oracle_and_fd():
+@pytest.mark.parametrize("schedule", SCHEDULES)
+def test_gtr100_full_size_vs_oracle_and_fd(schedule):
     """The 100-taxon 4-state GTR workload: 10,000 weighted patterns, 4 gamma
     categories; oracle parity and central finite differences (h = 1e-6)."""
     w = mcp.synthetic.make_workload("gtr100")
     assert w.C == 10_000 and w.N == 199
     blv = mcp.get_branchlength_vector(w.tree)
     llo, go = _oracle(w)
-    p = _plan(w)
+    p = _plan(w, schedule)
     ll, g = p.logl_grad(blv)
     assert rel_err(ll, llo) < LL_TOL
     assert grad_err(g, go) < GRAD_TOL
@@ -131,7 +137,8 @@ def test_multiple_phylodist_primates():
         mcp.MultiplePhyloDist([t, t], np.full((4, 3), 0.25), [1.0], [1.0], mcp.JC)
 
 
-def test_ragged_batch_vs_oracle():
+@pytest.mark.parametrize("schedule", SCHEDULES)
+def test_ragged_batch_vs_oracle(schedule):
     specs = [(201, 10, 2, 300, 4), (202, 40, 4, 1500, 4), (203, 25, 4, 640, 1),
              (204, 16, 2, 70, 2), (205, 31, 20, 200, 4)]
     ws = [mcp.synthetic.make_workload("custom", seed=s, n=n, S=S, C=C, R=R)
@@ -144,6 +151,7 @@ def test_ragged_batch_vs_oracle():
     # a second chain on dataset 1 with perturbed branch lengths
     items.append({"dataset": 1, "parent": ws[1].ta.parent, "n_rates": 4})
     b = mcp.Batch(datasets, items)
+    b.set_schedule(schedule)
     for i, w in enumerate(ws):
         b.set_model(i, mcp.Model(w.U, w.D, w.Uinv, w.mu, w.rates, w.pi))
     w1 = ws[1]
@@ -178,9 +186,10 @@ def _nni(ta):
     raise AssertionError("no internal edge")
 
 
-def test_topology_change_vs_oracle():
+@pytest.mark.parametrize("schedule", SCHEDULES)
+def test_topology_change_vs_oracle(schedule):
     w = mcp.synthetic.make_workload("custom", seed=301, n=30, S=4, C=400, R=2)
-    p = _plan(w)
+    p = _plan(w, schedule)
     blv = mcp.get_branchlength_vector(w.tree)
     par2 = _nni(w.ta)
     p.set_topology(par2)
@@ -192,9 +201,10 @@ def test_topology_change_vs_oracle():
     assert grad_err(g, go) < GRAD_TOL
 
 
-def test_graph_replays_match_eager():
+@pytest.mark.parametrize("schedule", SCHEDULES)
+def test_graph_replays_match_eager(schedule):
     w = mcp.synthetic.make_workload("custom", seed=401, n=50, S=4, C=2000, R=4)
-    p = _plan(w)
+    p = _plan(w, schedule)
     rng = np.random.default_rng(5)
     blv0 = mcp.get_branchlength_vector(w.tree)
     blvs = [blv0 * np.exp(rng.normal(0, 0.2, blv0.shape)) for _ in range(4)]
@@ -208,13 +218,17 @@ def test_graph_replays_match_eager():
     p.use_graph(False)
 
 
-def test_deterministic():
+def test_deterministic_and_schedules_agree():
     w = mcp.synthetic.make_workload("custom", seed=402, n=60, S=4, C=3000, R=4)
     p = _plan(w)
     b = mcp.get_branchlength_vector(w.tree)
     a1 = p.logl_grad(b)
     a2 = p.logl_grad(b)
     assert a1[0] == a2[0] and np.array_equal(a1[1], a2[1])
+    # the two schedules run the same tasks and the same reductions
+    p.set_schedule("levels")
+    a3 = p.logl_grad(b)
+    assert a3[0] == a1[0] and np.array_equal(a3[1], a1[1])
 
 
 def test_errors():
