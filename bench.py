@@ -48,24 +48,25 @@ def load_package():
 mcp = load_package()
 
 
-def make_batch_data(n_trees):
+def make_batch_data(n_trees, workers):
     """Ragged batch inputs, generated in worker processes started with
-    'spawn' before this process touches the GPU."""
+    'spawn' before this process touches the GPU (workers = 0: in-process)."""
+    if workers <= 0:
+        return mcp.synthetic.make_batched_workload(n_trees=n_trees)
     import multiprocessing as mp
     from concurrent.futures import ProcessPoolExecutor
     if mcp._lib._lib is not None:
         raise RuntimeError("batch data must be generated before the HIP library is loaded")
-    workers = max(1, min(8, (os.cpu_count() or 1)))
     with ProcessPoolExecutor(workers, mp_context=mp.get_context("spawn")) as pool:
         return mcp.synthetic.make_batched_workload(n_trees=n_trees, pool=pool)
 
 
-def bench_batched(data, steps, warmup, sample_trees):
+def bench_batched(data, steps, warmup, sample_trees, schedule):
     """Batched logL+grad over all (tree, chain) items in one engine, and the
     plain single-tree launch sequence on a sample of the same trees."""
     datasets, items, models, blv = data
     b = mcp.Batch(datasets, items)
-    b.set_schedule("fused")
+    b.set_schedule(schedule)
     for i, m in enumerate(models):
         b.set_model(i, mcp.Model(*m))
     n_items, total_patterns = b.sizes()
@@ -80,16 +81,21 @@ def bench_batched(data, steps, warmup, sample_trees):
     ms, ph = b.launch_times()
     b.set_timing(False)
     bytes_fwd, bytes_adj = b.algorithmic_bytes()
-    fused_ms = float(np.sum(ms[ph == 3]))
     b.close()
+    fwd_ms, adj_ms = float(np.sum(ms[ph == 2])), float(np.sum(ms[ph == 3]))
     out = {"items": n_items, "patterns_per_step": total_patterns,
-           "schedule": "fused",
+           "schedule": schedule,
            "ms_per_step": 1e3 * wall / steps,
            "value": total_patterns * steps / wall,
-           "fused_kernel_ms": fused_ms,
-           "fused_kernel_GBs": (bytes_fwd + bytes_adj) / (fused_ms * 1e-3) / 1e9,
            "algorithmic_GB_per_step": (bytes_fwd + bytes_adj) / 1e9,
            "launches": int(ms.shape[0])}
+    if schedule == "fused":
+        out["fused_kernel_ms"] = adj_ms
+        out["fused_kernel_GBs"] = (bytes_fwd + bytes_adj) / (adj_ms * 1e-3) / 1e9
+    else:
+        out.update({"post_order_ms": fwd_ms, "pre_order_ms": adj_ms,
+                    "post_order_GBs": bytes_fwd / (fwd_ms * 1e-3) / 1e9,
+                    "pre_order_GBs": bytes_adj / (adj_ms * 1e-3) / 1e9})
     # plain single-tree sequence (one launch per tree level) on the first
     # chain of the first trees
     pat, tsum = 0, 0.0
@@ -156,6 +162,10 @@ def main():
                          "is generated in spawned worker processes before the GPU is touched; "
                          "use 0 under a profiler that initialises the GPU at load")
     ap.add_argument("--batch-steps", type=int, default=5)
+    ap.add_argument("--batch-workers", type=int, default=8,
+                    help="spawned generator processes (0: generate in this process)")
+    ap.add_argument("--schedule", default="levels", choices=["levels", "fused"],
+                    help="launch schedule of the timed evaluations")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -166,7 +176,8 @@ def main():
     batch_data = None
     if args.batch_trees > 0:
         log(f"generating ragged batch ({args.batch_trees} trees x 4 chains)")
-        batch_data = make_batch_data(args.batch_trees)
+        batch_data = make_batch_data(args.batch_trees,
+                                     min(args.batch_workers, os.cpu_count() or 1))
     if mcp._lib.device_count() < 1:
         log("no HIP device visible")
         sys.exit(1)
@@ -175,27 +186,41 @@ def main():
     blv = mcp.get_branchlength_vector(w.tree)
     plan = mcp.Plan(w.ta.parent, w.x, len(w.rates), weights=w.weights)
     plan.set_model(mcp.Model(w.U, w.D, w.Uinv, w.mu, w.rates, w.pi))
-    plan.set_schedule("fused")
+    plan.set_schedule(args.schedule)
     for _ in range(args.warmup):
         plan.logl_grad(blv)
-    log(f"warmup done ({args.warmup}); timing {args.steps} steps")
+    log(f"warmup done ({args.warmup}); timing {args.steps} steps ({args.schedule} schedule)")
 
-    # timed region: eager launches, each kernel bracketed by HIP events
+    # timed region: eager launches, no instrumentation
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        plan.logl_grad(blv)
+    wall = time.perf_counter() - t0
+    value = w.C * args.steps / wall
+    log(f"eager: {1e3 * wall / args.steps:.3f} ms/step")
+
+    # the same steps again with every kernel bracketed by HIP events on the
+    # engine's stream: per-kernel durations for the roofline
     plan.set_timing(True)
-    phase_ms = np.zeros(5)
     launch_ms = None
     t0 = time.perf_counter()
     for _ in range(args.steps):
         plan.logl_grad(blv)
         ms, ph = plan.launch_times()
         launch_ms = ms if launch_ms is None else launch_ms + ms
-        phase_ms += plan.timing()
-    wall = time.perf_counter() - t0
-    phases = launch_ms.copy()
-    _, ph = plan.launch_times()
+    iwall = time.perf_counter() - t0
     plan.set_timing(False)
-    value = w.C * args.steps / wall
-    log(f"eager: {1e3 * wall / args.steps:.3f} ms/step")
+    bytes_fwd, bytes_adj = plan.algorithmic_bytes()
+    per_phase = {k: float(np.sum(launch_ms[ph == k])) / args.steps for k in (1, 2, 3, 4)}
+    n_phase = {k: int(np.sum(ph == k)) for k in (1, 2, 3, 4)}
+    if args.schedule == "fused":
+        kname = f"k_fused<{w.S},*,true>"
+        kbytes, kms, nl = bytes_fwd + bytes_adj, per_phase[3], n_phase[3]
+    elif per_phase[3] >= per_phase[2]:
+        kname, kbytes, kms, nl = f"k_level<{w.S},*,true> (pre-order)", bytes_adj, per_phase[3], n_phase[3]
+    else:
+        kname, kbytes, kms, nl = f"k_level<{w.S},*,false> (post-order)", bytes_fwd, per_phase[2], n_phase[2]
+    achieved = kbytes / (kms * 1e-3) / 1e9
 
     # graph replay of the same evaluation
     plan.use_graph(True)
@@ -209,38 +234,18 @@ def main():
     gvalue = w.C * args.steps / gwall
     log(f"graph: {1e3 * gwall / args.steps:.3f} ms/step")
 
-    bytes_fwd, bytes_adj = plan.algorithmic_bytes()
-    fused_ms = float(np.sum(phases[ph == 3])) / args.steps
-    kname, kbytes, kms, nl = "k_fused<4,2,true>", bytes_fwd + bytes_adj, fused_ms, int(np.sum(ph == 3))
-    achieved = kbytes / (kms * 1e-3) / 1e9
-
-    # the same evaluation with one launch per tree level
-    plan.set_schedule("levels")
+    # the other schedule, eager
+    other = "fused" if args.schedule == "levels" else "levels"
+    plan.set_schedule(other)
     for _ in range(3):
         plan.logl_grad(blv)
     t2 = time.perf_counter()
     for _ in range(args.steps):
         plan.logl_grad(blv)
-    lwall = time.perf_counter() - t2
-    plan.set_timing(True)
-    plan.logl_grad(blv)
-    lms, lph = plan.launch_times()
-    plan.set_timing(False)
-    plan.use_graph(True)
-    for _ in range(3):
-        plan.logl_grad(blv)
-    t3 = time.perf_counter()
-    for _ in range(args.steps):
-        plan.logl_grad(blv)
-    lgwall = time.perf_counter() - t3
-    plan.use_graph(False)
-    levels = {"value": w.C * args.steps / lwall, "graph_value": w.C * args.steps / lgwall,
-              "ms_per_step": 1e3 * lwall / args.steps,
-              "post_order_ms": float(np.sum(lms[lph == 2])),
-              "pre_order_ms": float(np.sum(lms[lph == 3])),
-              "launches": int(lms.shape[0])}
-    log(f"levels schedule: {1e3 * lwall / args.steps:.3f} ms/step eager, "
-        f"{1e3 * lgwall / args.steps:.3f} graph")
+    owall = time.perf_counter() - t2
+    plan.set_schedule(args.schedule)
+    log(f"{other} schedule: {1e3 * owall / args.steps:.3f} ms/step")
+
     result = {
         "metric": "logL+grad site-pattern evals/sec",
         "value": value,
@@ -262,18 +267,21 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": kname, "launches_per_step": nl,
                      "algorithmic_bytes_per_step": kbytes,
-                     "kernel_ms_per_step": kms},
-        "schedule": "fused",
+                     "kernel_ms_per_step": kms,
+                     "avg_launch_us": 1e3 * kms / max(nl, 1)},
+        "schedule": args.schedule,
         "graph_value": gvalue,
-        "phase_ms_per_step": {"P_build": float(np.sum(phases[ph == 1])) / args.steps,
-                              "fused_fwd_adj": fused_ms,
-                              "reduce": float(np.sum(phases[ph == 4])) / args.steps},
+        "instrumented_ms_per_step": 1e3 * iwall / args.steps,
+        "kernel_ms_per_step": {"P_build": per_phase[1], "post_order": per_phase[2],
+                               "pre_order": per_phase[3], "reduce": per_phase[4]},
+        "launches_per_step": n_phase,
         "algorithmic_bytes_per_step": {"post_order": bytes_fwd, "pre_order": bytes_adj},
-        "levels_schedule": levels,
+        "other_schedule": {"schedule": other, "value": w.C * args.steps / owall,
+                           "ms_per_step": 1e3 * owall / args.steps},
     }
     if batch_data is not None:
         log("timing ragged batch")
-        result["batched"] = bench_batched(batch_data, args.batch_steps, 2, 16)
+        result["batched"] = bench_batched(batch_data, args.batch_steps, 2, 16, args.schedule)
         result["batched"]["workload"] = (
             f"{args.batch_trees} trees, taxa ~ U{{16..512}}, patterns ~ LogU[500, 20000], "
             "restriction/GTR-4 50/50, 4 gamma cats, 4 chains per tree, FP64 logL+grad")

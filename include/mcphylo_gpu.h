@@ -149,11 +149,10 @@ int mcp_use_graph(mcp_batch* batch, int32_t on);
 
 /* Launch schedule of an evaluation.
  *  MCP_SCHEDULE_FUSED   one launch walks every level of a (tree, category,
- *                       pattern tile) with block barriers between levels
- *                       (default);
+ *                       pattern tile) with block barriers between levels;
  *  MCP_SCHEDULE_LEVELS  one launch per post-order level (nodes of equal
  *                       height) and per pre-order level (nodes of equal
- *                       depth), over every item. */
+ *                       depth), over every item (default). */
 #define MCP_SCHEDULE_FUSED 0
 #define MCP_SCHEDULE_LEVELS 1
 int mcp_set_schedule(mcp_batch* batch, int32_t mode);

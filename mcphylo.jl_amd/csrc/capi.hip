@@ -178,7 +178,7 @@ struct Launch {
 struct mcp_engine {
     int device = 0;
     int waves = 4;     // wavefronts per block of the fused kernel
-    int schedule = MCP_SCHEDULE_FUSED;
+    int schedule = MCP_SCHEDULE_LEVELS;
     int num_cu = 256;
     hipStream_t stream = nullptr;
     std::vector<DatasetHost> ds;
