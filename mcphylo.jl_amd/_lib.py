@@ -14,7 +14,8 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libmcphylo_hip.so")
+# MCPHYLO_HIP_LIB selects another build of the same library (e.g. a tuned variant)
+LIB_PATH = os.environ.get("MCPHYLO_HIP_LIB") or os.path.join(_HERE, "libmcphylo_hip.so")
 CSRC = os.path.join(_HERE, "csrc")
 
 MCP_OK = 0

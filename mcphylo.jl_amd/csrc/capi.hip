@@ -26,7 +26,8 @@
 namespace mcp {
 hipError_t launch_build_P(const ModelDev*, const int32_t*, int, long long, hipStream_t);
 hipError_t launch_tip_tables(const ModelDev*, const int32_t*, int, long long, hipStream_t);
-hipError_t launch_level(bool, int, const LevelJob*, const int32_t*, int, double*, int, hipStream_t);
+hipError_t launch_level(bool, int, const LevelJob*, const int32_t*, const int32_t*, int, int, int,
+                        double*, hipStream_t);
 hipError_t launch_fused(bool, int, const ItemSched*, int, int, int, const LevelJob*, const int32_t*,
                         const LevelJob*, const int32_t*, double*, hipStream_t);
 int cols_per_thread(int S);
@@ -35,6 +36,12 @@ bool supported_states(int S);
 }  // namespace mcp
 
 using namespace mcp;
+
+// Persistent grid of the level kernels: at most this many single-wave blocks
+// per compute unit.
+#ifndef MCP_GRID_PER_CU
+#define MCP_GRID_PER_CU 256
+#endif
 
 namespace {
 
@@ -171,6 +178,8 @@ struct Launch {
     int job_begin, njobs;
     int taskoff_begin;  // index into task_off arena (njobs+1 entries)
     int ntasks;
+    int grid, chunk;    // blocks and tasks per block
+    int blk_begin;      // index into the block -> first job arena (grid entries)
 };
 
 }  // namespace
@@ -186,11 +195,11 @@ struct mcp_engine {
     // device arenas
     DevBuf dF, dG0, dG1, dP, ddP, dPT, ddPT, dparts, dout, dblv, dmodel, dcodes, dweights, dtiptab,
         dleafof, dmodels, djobs_f, djobs_a, dmore, dtaskoff, dred, delemP, delemT, dsched, dlvl_f,
-        dlvl_a;
+        dlvl_a, dblkjob;
     PinnedBuf hblv, hout;
     // schedule
     std::vector<Launch> fwd, adj;
-    std::vector<int32_t> taskoff;
+    std::vector<int32_t> taskoff, blkjob;
     struct FusedLaunch {
         int S, sched_begin, nitems, nblocks;
     };
@@ -629,6 +638,7 @@ static int rebuild_schedule(mcp_engine* e) {
     // level schedule: post-order by height, pre-order by depth, per state count
     std::vector<LevelJob> jf, ja;
     e->taskoff.clear();
+    e->blkjob.clear();
     e->fwd.clear();
     e->adj.clear();
     e->fused.clear();
@@ -673,6 +683,19 @@ static int rebuild_schedule(mcp_engine* e) {
             return MCP_OK;
         }
         if (tasks > 0x7fffffffLL) return fail(MCP_ERR_UNSUPPORTED, "too many tasks in one level");
+        // persistent grid: chunks of consecutive tasks, no empty block
+        const int maxgrid = e->num_cu * MCP_GRID_PER_CU;
+        int grid = (int)std::min<long long>(tasks, maxgrid);
+        L.chunk = (int)((tasks + grid - 1) / grid);
+        L.grid = (int)((tasks + L.chunk - 1) / L.chunk);
+        L.blk_begin = (int)e->blkjob.size();
+        const int32_t* off = e->taskoff.data() + L.taskoff_begin;
+        int j = 0;
+        for (int b = 0; b < L.grid; ++b) {
+            const long long t0 = (long long)b * L.chunk;
+            while (off[j + 1] <= t0) ++j;
+            e->blkjob.push_back(j);
+        }
         (adjoint ? e->adj : e->fwd).push_back(L);
         return MCP_OK;
     };
@@ -744,6 +767,8 @@ static int rebuild_schedule(mcp_engine* e) {
     if (!more.empty() && (st = upload(e, e->dmore, more.data(), more.size() * sizeof(ChildRef))))
         return st;
     if ((st = upload(e, e->dtaskoff, e->taskoff.data(), e->taskoff.size() * sizeof(int32_t))))
+        return st;
+    if ((st = upload(e, e->dblkjob, e->blkjob.data(), e->blkjob.size() * sizeof(int32_t))))
         return st;
 
     // reductions: ll of every item first, then every branch gradient
@@ -958,17 +983,18 @@ static int enqueue_eval(mcp_engine* e, bool grad) {
                             (int)e->items.size(), e->nelemP, st));
     TIMED(1, launch_tip_tables(e->dmodels.as<ModelDev>(), e->delemT.as<int32_t>(),
                                (int)e->items.size(), e->nelemT, st));
-    const int maxgrid = e->num_cu * 32;
     for (const Launch& L : e->fwd) {
         TIMED(2, launch_level(false, L.S, e->djobs_f.as<LevelJob>() + L.job_begin,
-                              e->dtaskoff.as<int32_t>() + L.taskoff_begin, L.njobs,
-                              e->dparts.as<double>(), std::min(L.ntasks, maxgrid), st));
+                              e->dtaskoff.as<int32_t>() + L.taskoff_begin,
+                              e->dblkjob.as<int32_t>() + L.blk_begin, L.ntasks, L.chunk, L.grid,
+                              e->dparts.as<double>(), st));
     }
     if (grad)
         for (const Launch& L : e->adj) {
             TIMED(3, launch_level(true, L.S, e->djobs_a.as<LevelJob>() + L.job_begin,
-                                  e->dtaskoff.as<int32_t>() + L.taskoff_begin, L.njobs,
-                                  e->dparts.as<double>(), std::min(L.ntasks, maxgrid), st));
+                                  e->dtaskoff.as<int32_t>() + L.taskoff_begin,
+                                  e->dblkjob.as<int32_t>() + L.blk_begin, L.ntasks, L.chunk,
+                                  L.grid, e->dparts.as<double>(), st));
         }
     for (const auto& F : e->fused) {
         TIMED(grad ? 3 : 2,

@@ -63,13 +63,29 @@ __device__ __forceinline__ double block_sum(double v, double* lds) {
     return s;
 }
 
+// Pointers read out of job records are generic; casting them to the global
+// address space lets the compiler emit global (not flat) loads and stores,
+// which wait on the vector-memory counter only.
+#define MCP_GLOBAL __attribute__((address_space(1)))
+
+template <class T>
+__device__ __forceinline__ const MCP_GLOBAL T* gptr(const T* p) {
+    return (const MCP_GLOBAL T*)p;
+}
+template <class T>
+__device__ __forceinline__ MCP_GLOBAL T* gptr_w(T* p) {
+    return (MCP_GLOBAL T*)p;
+}
+
+typedef double dvec2 __attribute__((ext_vector_type(2)));
+
 template <int S>
-__device__ __forceinline__ void load_row(const double* __restrict__ p, double (&v)[S]) {
+__device__ __forceinline__ void load_row(const MCP_GLOBAL double* __restrict__ p, double (&v)[S]) {
     if constexpr (S % 2 == 0) {
-        const double2* q = reinterpret_cast<const double2*>(p);
+        const MCP_GLOBAL dvec2* q = reinterpret_cast<const MCP_GLOBAL dvec2*>(p);
 #pragma unroll
         for (int i = 0; i < S / 2; ++i) {
-            double2 t = q[i];
+            const dvec2 t = q[i];
             v[2 * i] = t.x;
             v[2 * i + 1] = t.y;
         }
@@ -80,11 +96,16 @@ __device__ __forceinline__ void load_row(const double* __restrict__ p, double (&
 }
 
 template <int S>
-__device__ __forceinline__ void store_row(double* __restrict__ p, const double (&v)[S]) {
+__device__ __forceinline__ void store_row(MCP_GLOBAL double* __restrict__ p, const double (&v)[S]) {
     if constexpr (S % 2 == 0) {
-        double2* q = reinterpret_cast<double2*>(p);
+        MCP_GLOBAL dvec2* q = reinterpret_cast<MCP_GLOBAL dvec2*>(p);
 #pragma unroll
-        for (int i = 0; i < S / 2; ++i) q[i] = make_double2(v[2 * i], v[2 * i + 1]);
+        for (int i = 0; i < S / 2; ++i) {
+            dvec2 t;
+            t.x = v[2 * i];
+            t.y = v[2 * i + 1];
+            q[i] = t;
+        }
     } else {
 #pragma unroll
         for (int i = 0; i < S; ++i) p[i] = v[i];
@@ -181,35 +202,41 @@ __device__ __forceinline__ const ChildRef& child_of(const LevelJob& J, int q) {
     return q < kInlineChildren ? J.kid[q] : J.more[q - kInlineChildren];
 }
 
+// Row `code` of a wave-uniform table tab[K][S] for each lane: a gather from
+// a table of a few hundred bytes that stays in the L1/L2 caches.
+template <int S>
+__device__ __forceinline__ void select_row(const MCP_GLOBAL double* __restrict__ tab, int code,
+                                           double (&out)[S]) {
+    const MCP_GLOBAL double* row = tab + code * S;
+#pragma unroll
+    for (int s = 0; s < S; ++s) out[s] = row[s];
+}
+
+// Columns of a task: lane l owns c0 + 64 k (k < CPT), c0 = tile base + l.
+// Slabs, codes and weights are allocated over the padded pattern range, so
+// loads and stores of padding columns stay in bounds and are issued
+// unconditionally; only the sums mask them (select, so padding garbage never
+// reaches a result).
+
 // Message each of a thread's CPT columns receives from child `ch` in
 // category r: msg[s] = sum_s1 P[s][s1] f[s1] (internal child, f loaded into
 // `f`), or the leaf's precomputed row PT[r][code] (`f` untouched).
 template <int S, int CPT>
 __device__ __forceinline__ void child_fetch(const ChildRef& ch, const LevelJob& J, int r, int c0,
-                                            int bd, double (&f)[CPT][S], double (&m)[CPT][S]) {
+                                            double (&f)[CPT][S], double (&m)[CPT][S]) {
     if (ch.codes) {
-        const double* pt = ch.P + (size_t)r * J.K * S;
+        const MCP_GLOBAL double* pt = gptr(ch.P) + (size_t)r * J.K * S;
+        const MCP_GLOBAL uint8_t* codes = gptr(ch.codes);
+        int code[CPT];
 #pragma unroll
-        for (int k = 0; k < CPT; ++k) {
-            const int c = c0 + k * bd;
-            const int code = c < J.C ? (int)ch.codes[c] : 0;
-            const double* row = pt + code * S;
+        for (int k = 0; k < CPT; ++k) code[k] = (int)codes[c0 + 64 * k];
 #pragma unroll
-            for (int s = 0; s < S; ++s) m[k][s] = row[s];
-        }
+        for (int k = 0; k < CPT; ++k) select_row<S>(pt, code[k], m[k]);
     } else {
-        const double* src = ch.F + (size_t)r * J.Cp * S;
+        const MCP_GLOBAL double* src = gptr(ch.F) + (size_t)r * J.Cp * S;
 #pragma unroll
-        for (int k = 0; k < CPT; ++k) {
-            const int c = c0 + k * bd;
-            if (c < J.C) {
-                load_row<S>(src + (size_t)c * S, f[k]);
-            } else {
-#pragma unroll
-                for (int s = 0; s < S; ++s) f[k][s] = 1.0;
-            }
-        }
-        const double* __restrict__ P = ch.P + (size_t)r * (S * S);
+        for (int k = 0; k < CPT; ++k) load_row<S>(src + (size_t)(c0 + 64 * k) * S, f[k]);
+        const MCP_GLOBAL double* __restrict__ P = gptr(ch.P) + (size_t)r * (S * S);
 #pragma unroll
         for (int k = 0; k < CPT; ++k)
 #pragma unroll
@@ -226,9 +253,8 @@ __device__ __forceinline__ void child_fetch(const ChildRef& ch, const LevelJob& 
 template <int S, int CPT>
 __device__ __forceinline__ void fwd_task(const LevelJob& J, int r, int tile,
                                          double* __restrict__ parts) {
-    constexpr int bd = 64;
     const int lane = (int)(threadIdx.x & 63);
-    const int c0 = tile * bd * CPT + lane;
+    const int c0 = tile * 64 * CPT + lane;
     double acc[CPT][S];
 #pragma unroll
     for (int k = 0; k < CPT; ++k)
@@ -236,33 +262,41 @@ __device__ __forceinline__ void fwd_task(const LevelJob& J, int r, int tile,
         for (int s = 0; s < S; ++s) acc[k][s] = 1.0;
     for (int q = 0; q < J.nchild; ++q) {
         double f[CPT][S], m[CPT][S];
-        child_fetch<S, CPT>(child_of(J, q), J, r, c0, bd, f, m);
+        child_fetch<S, CPT>(child_of(J, q), J, r, c0, f, m);
 #pragma unroll
         for (int k = 0; k < CPT; ++k)
 #pragma unroll
             for (int s = 0; s < S; ++s) acc[k][s] *= m[k][s];
     }
+    const MCP_GLOBAL double* wts = gptr(J.weights);
     double contrib = 0.0;
+    if (J.is_root) {
+        // root_sum: log sum_s pi_s Ftilde_s (VectorizedFunctions.jl:76-87)
+        const MCP_GLOBAL double* pi = gptr(J.pi);
 #pragma unroll
-    for (int k = 0; k < CPT; ++k) {
-        const int c = c0 + k * bd;
-        if (c >= J.C) continue;
-        if (J.is_root) {
-            // root_sum: log sum_s pi_s Ftilde_s (VectorizedFunctions.jl:76-87)
+        for (int k = 0; k < CPT; ++k) {
+            const int c = c0 + 64 * k;
             double t = 0.0;
 #pragma unroll
-            for (int s = 0; s < S; ++s) t = fma(acc[k][s], J.pi[s], t);
-            contrib += J.weights[c] * log(t);
-        } else {
-            // by_max!: rescale by the state maximum, keep its log (VectorizedFunctions.jl:31-49)
+            for (int s = 0; s < S; ++s) t = fma(acc[k][s], pi[s], t);
+            const double v = wts[c] * log(t);
+            contrib += c < J.C ? v : 0.0;
+        }
+    } else {
+        // by_max!: rescale by the state maximum, keep its log (VectorizedFunctions.jl:31-49)
+        MCP_GLOBAL double* dst = gptr_w(J.F) + (size_t)r * J.Cp * S;
+#pragma unroll
+        for (int k = 0; k < CPT; ++k) {
+            const int c = c0 + 64 * k;
             double mx = -INFINITY;
 #pragma unroll
             for (int s = 0; s < S; ++s) mx = (mx < acc[k][s]) ? acc[k][s] : mx;
             const double inv = 1.0 / mx;
 #pragma unroll
             for (int s = 0; s < S; ++s) acc[k][s] *= inv;
-            store_row<S>(J.F + ((size_t)r * J.Cp + c) * S, acc[k]);
-            contrib += J.weights[c] * log(mx);
+            store_row<S>(dst + (size_t)c * S, acc[k]);
+            const double v = wts[c] * log(mx);
+            contrib += c < J.C ? v : 0.0;
         }
     }
     const double bs = wave_sum(contrib);
@@ -279,33 +313,35 @@ __device__ __forceinline__ void fwd_task(const LevelJob& J, int r, int tile,
 // products of T_u with the leaf's dP.tip and P.tip rows.
 template <int S, int CPT>
 __device__ __forceinline__ double child_adjoint(const ChildRef& ch, const LevelJob& J, int r, int c0,
-                                                int bd, const double (&T)[CPT][S],
+                                                const double (&T)[CPT][S],
                                                 const double (&f)[CPT][S],
                                                 const double (&m)[CPT][S]) {
+    const MCP_GLOBAL double* wts = gptr(J.weights);
     double contrib = 0.0;
     if (ch.codes) {
-        const double* dpt = ch.dP + (size_t)r * J.K * S;
+        const MCP_GLOBAL double* dpt = gptr(ch.dP) + (size_t)r * J.K * S;
+        const MCP_GLOBAL uint8_t* codes = gptr(ch.codes);
 #pragma unroll
         for (int k = 0; k < CPT; ++k) {
-            const int c = c0 + k * bd;
-            if (c >= J.C) continue;
-            const double* row = dpt + (int)ch.codes[c] * S;
+            const int c = c0 + 64 * k;
+            double row[S];
+            select_row<S>(dpt, (int)codes[c], row);
             double num = 0.0, col = 0.0;
 #pragma unroll
             for (int s = 0; s < S; ++s) {
                 num = fma(T[k][s], row[s], num);
                 col = fma(T[k][s], m[k][s], col);
             }
-            contrib += J.weights[c] * (num / col);
+            const double v = wts[c] * (num / col);
+            contrib += c < J.C ? v : 0.0;
         }
     } else {
-        const double* __restrict__ P = ch.P + (size_t)r * (S * S);
-        const double* __restrict__ dP = ch.dP + (size_t)r * (S * S);
-        double* Gdst = ch.G + (size_t)r * J.Cp * S;
+        const MCP_GLOBAL double* __restrict__ P = gptr(ch.P) + (size_t)r * (S * S);
+        const MCP_GLOBAL double* __restrict__ dP = gptr(ch.dP) + (size_t)r * (S * S);
+        MCP_GLOBAL double* Gdst = gptr_w(ch.G) + (size_t)r * J.Cp * S;
 #pragma unroll
         for (int k = 0; k < CPT; ++k) {
-            const int c = c0 + k * bd;
-            if (c >= J.C) continue;
+            const int c = c0 + 64 * k;
             double g[S];
 #pragma unroll
             for (int s = 0; s < S; ++s) g[s] = 0.0;
@@ -330,7 +366,8 @@ __device__ __forceinline__ double child_adjoint(const ChildRef& ch, const LevelJ
 #pragma unroll
             for (int s = 0; s < S; ++s) g[s] *= inv;
             store_row<S>(Gdst + (size_t)c * S, g);
-            contrib += J.weights[c] * (num / col);
+            const double v = wts[c] * (num / col);
+            contrib += c < J.C ? v : 0.0;
         }
     }
     return contrib;
@@ -339,44 +376,36 @@ __device__ __forceinline__ double child_adjoint(const ChildRef& ch, const LevelJ
 template <int S, int CPT>
 __device__ __forceinline__ void adj_task(const LevelJob& J, int r, int tile,
                                          double* __restrict__ parts) {
-    constexpr int bd = 64;
     const int lane = (int)(threadIdx.x & 63);
-    const int c0 = tile * bd * CPT + lane;
+    const int c0 = tile * 64 * CPT + lane;
     double gp[CPT][S];
     if (J.G) {
-        const double* src = J.G + (size_t)r * J.Cp * S;
+        const MCP_GLOBAL double* src = gptr(J.G) + (size_t)r * J.Cp * S;
 #pragma unroll
-        for (int k = 0; k < CPT; ++k) {
-            const int c = c0 + k * bd;
-            if (c < J.C) {
-                load_row<S>(src + (size_t)c * S, gp[k]);
-            } else {
-#pragma unroll
-                for (int s = 0; s < S; ++s) gp[k][s] = 1.0;
-            }
-        }
+        for (int k = 0; k < CPT; ++k) load_row<S>(src + (size_t)(c0 + 64 * k) * S, gp[k]);
     } else {
+        const MCP_GLOBAL double* pi = gptr(J.pi);
 #pragma unroll
         for (int k = 0; k < CPT; ++k)
 #pragma unroll
-            for (int s = 0; s < S; ++s) gp[k][s] = J.pi[s];
+            for (int s = 0; s < S; ++s) gp[k][s] = pi[s];
     }
     if (J.nchild == 2) {
         const ChildRef& a = J.kid[0];
         const ChildRef& b = J.kid[1];
         double fa[CPT][S], ma[CPT][S], fb[CPT][S], mb[CPT][S], T[CPT][S];
-        child_fetch<S, CPT>(a, J, r, c0, bd, fa, ma);
-        child_fetch<S, CPT>(b, J, r, c0, bd, fb, mb);
+        child_fetch<S, CPT>(a, J, r, c0, fa, ma);
+        child_fetch<S, CPT>(b, J, r, c0, fb, mb);
 #pragma unroll
         for (int k = 0; k < CPT; ++k)
 #pragma unroll
             for (int s = 0; s < S; ++s) T[k][s] = gp[k][s] * mb[k][s];
-        const double ca = child_adjoint<S, CPT>(a, J, r, c0, bd, T, fa, ma);
+        const double ca = child_adjoint<S, CPT>(a, J, r, c0, T, fa, ma);
 #pragma unroll
         for (int k = 0; k < CPT; ++k)
 #pragma unroll
             for (int s = 0; s < S; ++s) T[k][s] = gp[k][s] * ma[k][s];
-        const double cb = child_adjoint<S, CPT>(b, J, r, c0, bd, T, fb, mb);
+        const double cb = child_adjoint<S, CPT>(b, J, r, c0, T, fb, mb);
         const double sa = wave_sum(ca);
         const double sb = wave_sum(cb);
         if (lane == 0) {
@@ -396,14 +425,14 @@ __device__ __forceinline__ void adj_task(const LevelJob& J, int r, int tile,
         for (int q2 = 0; q2 < J.nchild; ++q2) {
             if (q2 == q) continue;
             double fw[CPT][S], mw[CPT][S];
-            child_fetch<S, CPT>(child_of(J, q2), J, r, c0, bd, fw, mw);
+            child_fetch<S, CPT>(child_of(J, q2), J, r, c0, fw, mw);
 #pragma unroll
             for (int k = 0; k < CPT; ++k)
 #pragma unroll
                 for (int s = 0; s < S; ++s) T[k][s] *= mw[k][s];
         }
-        child_fetch<S, CPT>(u, J, r, c0, bd, fu, mu_);
-        const double cu = child_adjoint<S, CPT>(u, J, r, c0, bd, T, fu, mu_);
+        child_fetch<S, CPT>(u, J, r, c0, fu, mu_);
+        const double cu = child_adjoint<S, CPT>(u, J, r, c0, T, fu, mu_);
         const double su = wave_sum(cu);
         if (lane == 0) parts[u.gpart + r * J.ntiles + tile] = su;
     }
@@ -411,17 +440,22 @@ __device__ __forceinline__ void adj_task(const LevelJob& J, int r, int tile,
 
 // One launch = one level (post-order: nodes of one height; pre-order: nodes
 // of one depth) of every item with S states.  Blocks are single wavefronts;
-// block b takes the contiguous task chunk [b*chunk, (b+1)*chunk).
+// block b takes the contiguous task chunk [b*chunk, (b+1)*chunk) and starts
+// at job blk_job[b] (computed on the host), so a block reaches its first
+// operands after two rounds of scalar loads.
+#ifndef MCP_ADJ_WAVES_PER_EU
+#define MCP_ADJ_WAVES_PER_EU 1
+#endif
+
 template <int S, int CPT, bool ADJ>
-__global__ void __launch_bounds__(64) k_level(const LevelJob* __restrict__ jobs,
-                                              const int32_t* __restrict__ task_off, int njobs,
-                                              double* __restrict__ parts) {
-    const int total = task_off[njobs];
-    const int chunk = (total + gridDim.x - 1) / gridDim.x;
+__global__ void __launch_bounds__(64)
+__attribute__((amdgpu_waves_per_eu(ADJ ? MCP_ADJ_WAVES_PER_EU : 1))) k_level(const LevelJob* __restrict__ jobs,
+                                              const int32_t* __restrict__ task_off,
+                                              const int32_t* __restrict__ blk_job, int total,
+                                              int chunk, double* __restrict__ parts) {
     const int t0 = blockIdx.x * chunk;
     const int t1 = min(total, t0 + chunk);
-    if (t0 >= t1) return;
-    int j = find_job(task_off, njobs, t0);
+    int j = blk_job[blockIdx.x];
     int jbeg = task_off[j], jend = task_off[j + 1];
     for (int t = t0; t < t1; ++t) {
         while (t >= jend) {
@@ -531,7 +565,15 @@ hipError_t launch_tip_tables(const ModelDev* models, const int32_t* elem_off, in
 }
 
 // State counts with built kernels and the columns per thread used for each.
-#define MCP_FOR_EACH_S(X) X(1, 2) X(2, 4) X(3, 2) X(4, 2) X(5, 1) X(6, 1) X(7, 1) X(8, 1) X(16, 1) X(20, 1)
+#ifndef MCP_CPT_S2
+#define MCP_CPT_S2 4
+#endif
+#ifndef MCP_CPT_S4
+#define MCP_CPT_S4 2
+#endif
+#define MCP_FOR_EACH_S(X)                                                                  \
+    X(1, 2) X(2, MCP_CPT_S2) X(3, 2) X(4, MCP_CPT_S4) X(5, 1) X(6, 1) X(7, 1) X(8, 1) X(16, 1) \
+    X(20, 1)
 
 bool supported_states(int S) {
     switch (S) {
@@ -552,17 +594,18 @@ int cols_per_thread(int S) {
 }
 
 hipError_t launch_level(bool adjoint, int S, const LevelJob* jobs, const int32_t* task_off,
-                        int njobs, double* parts, int grid, hipStream_t st) {
-    if (njobs <= 0 || grid <= 0) return hipSuccess;
+                        const int32_t* blk_job, int total, int chunk, int grid, double* parts,
+                        hipStream_t st) {
+    if (total <= 0 || grid <= 0) return hipSuccess;
     switch (S) {
 #define X(n, c)                                                                                  \
     case n:                                                                                      \
         if (adjoint)                                                                             \
             hipLaunchKernelGGL((k_level<n, c, true>), dim3(grid), dim3(64), 0, st, jobs,         \
-                               task_off, njobs, parts);                                          \
+                               task_off, blk_job, total, chunk, parts);                          \
         else                                                                                     \
             hipLaunchKernelGGL((k_level<n, c, false>), dim3(grid), dim3(64), 0, st, jobs,        \
-                               task_off, njobs, parts);                                          \
+                               task_off, blk_job, total, chunk, parts);                          \
         break;
         MCP_FOR_EACH_S(X)
 #undef X
