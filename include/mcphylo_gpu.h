@@ -139,6 +139,26 @@ int mcp_batch_logl(mcp_batch* batch, const double* blv_concat, double* ll);
 int mcp_batch_logl_grad(mcp_batch* batch, const double* blv_concat, double* ll,
                         double* grad_concat);
 
+/* ---- device leapfrog (NUTS / HMC over branch lengths) ------------------- */
+
+/* Branch-length prior added to the log posterior of item `item` by the
+ * leapfrog: kind 0 none, kind 1 exponentialBL(scale) -- each branch
+ * Exponential with mean `scale` (src/Likelihood/Prior.jl:50-56,80-83). */
+int mcp_hmc_set_prior(mcp_batch* batch, int32_t item, int32_t kind, double scale);
+
+/* One leapfrog step for every item, as refraction! does it
+ * (src/samplers/tree_hamiltonian/refraction.jl:2-31; reflection at zero in
+ * time order as in ref_NNI!, :35-91, without the NNI topology attempt):
+ *   r += eps/2 g;  y = x + eps r, reflected at zero;  x = molifier(y, delta);
+ *   (ll, grad) = logL+grad(x);  g = (grad + dlogprior) * scale_fac(x, delta);
+ *   r += eps/2 g;  logpost = ll + logprior(x)
+ * (molifier / scale_fac: src/samplers/tree_hamiltonian/auxilliary.jl:113-119).
+ * eps_delta: [2*n_items] (eps, delta) pairs; x, r, g: in/out, blv_concat
+ * layout; logpost: out [n_items]; ll: out [n_items] or NULL.  The whole step
+ * is one HIP graph replay when mcp_use_graph is on. */
+int mcp_hmc_leapfrog(mcp_batch* batch, const double* eps_delta, double* x, double* r, double* g,
+                     double* logpost, double* ll);
+
 /* ---- launch control ----------------------------------------------------- */
 
 /* Capture the whole evaluation (upload of branch lengths, P/dP build, every

@@ -33,6 +33,10 @@ hipError_t launch_fused(bool, int, const ItemSched*, int, int, int, const LevelJ
 int cols_per_thread(int S);
 hipError_t launch_reduce(const RedOp*, int, const double*, double*, hipStream_t);
 bool supported_states(int S);
+hipError_t launch_lf_pre(const HmcItem*, int, double*, double*, const double*, const double*,
+                         double*, int, hipStream_t);
+hipError_t launch_lf_post(const HmcItem*, int, const double*, double*, double*, const double*,
+                          const double*, double*, hipStream_t);
 }  // namespace mcp
 
 using namespace mcp;
@@ -212,7 +216,12 @@ struct mcp_engine {
     double bytes_fwd = 0.0, bytes_adj = 0.0;
     // graphs
     bool use_graph = false;
-    hipGraphExec_t graph_exec[2] = {nullptr, nullptr};  // [0] logl, [1] logl+grad
+    hipGraphExec_t graph_exec[3] = {nullptr, nullptr, nullptr};  // logl, logl+grad, leapfrog
+    // device leapfrog: [x | r | g] per item (blv layout), eps/delta pairs, log posterior
+    DevBuf dhmc, dhmc_items, dlp;
+    PinnedBuf hhmc;
+    std::vector<HmcItem> hmc_items;
+    bool hmc_items_dirty = true;
     // timing: one event pair around every kernel launch of an evaluation
     bool timing = false;
     hipEvent_t ev_begin = nullptr, ev_end = nullptr;
@@ -491,6 +500,7 @@ static int upload(mcp_engine* e, DevBuf& b, const void* src, size_t bytes) {
 // rebuilt with fresh addresses.
 static int rebuild_schedule(mcp_engine* e) {
     e->drop_graphs();
+    e->hmc_items_dirty = true;
     const int nitems = (int)e->items.size();
     // pre-order ring buffers
     size_t g0 = 0, g1 = 0;
@@ -1070,6 +1080,128 @@ static int engine_eval(mcp_engine* e, const double* blv, double* ll, double* gra
 }
 
 // ---------------------------------------------------------------------------
+// device leapfrog (NUTS / HMC step over branch lengths)
+// ---------------------------------------------------------------------------
+
+static int hmc_prepare(mcp_engine* e) {
+    const int n = (int)e->items.size();
+    if (e->hmc_items.size() != (size_t)n) {
+        e->hmc_items.assign(n, HmcItem{});
+        e->hmc_items_dirty = true;
+    }
+    if (!e->hmc_items_dirty) return MCP_OK;
+    for (int i = 0; i < n; ++i) {
+        HmcItem& h = e->hmc_items[i];
+        h.n = e->items[i].N - 1;
+        h.off = (int32_t)e->items[i].blv_off;
+        h.out = e->items[i].out_off;
+    }
+    int st;
+    if ((st = upload(e, e->dhmc_items, e->hmc_items.data(), n * sizeof(HmcItem)))) return st;
+    const size_t T = e->total_blv;
+    if ((st = e->dhmc.ensure((3 * T + 2 * (size_t)n) * sizeof(double)))) return st;
+    if ((st = e->dlp.ensure((size_t)n * sizeof(double)))) return st;
+    if ((st = e->hhmc.ensure((3 * T + 2 * (size_t)n + n + e->total_out) * sizeof(double)))) return st;
+    e->hmc_items_dirty = false;
+    e->drop_graphs();
+    return MCP_OK;
+}
+
+// H2D [x|r|g|eps,delta] -> k_lf_pre -> P, tips, levels -> reduce -> k_lf_post
+// -> D2H [x|r|g], lp, (ll, grad) outputs
+static int enqueue_leapfrog(mcp_engine* e) {
+    hipStream_t st = e->stream;
+    const size_t T = e->total_blv;
+    const int n = (int)e->items.size();
+    double* dx = e->dhmc.as<double>();
+    double* dr = dx + T;
+    double* dg = dr + T;
+    double* ded = dg + T;
+    double* h = e->hhmc.as<double>();
+    HIP_TRY(hipMemcpyAsync(dx, h, (3 * T + 2 * (size_t)n) * sizeof(double), hipMemcpyHostToDevice,
+                           st));
+    HIP_TRY(launch_lf_pre(e->dhmc_items.as<HmcItem>(), n, dx, dr, dg, ded, e->dblv.as<double>(),
+                          64 * (int)std::max<size_t>(1, T), st));
+    HIP_TRY(launch_build_P(e->dmodels.as<ModelDev>(), e->delemP.as<int32_t>(), n, e->nelemP, st));
+    HIP_TRY(launch_tip_tables(e->dmodels.as<ModelDev>(), e->delemT.as<int32_t>(), n, e->nelemT,
+                              st));
+    for (const Launch& L : e->fwd)
+        HIP_TRY(launch_level(false, L.S, e->djobs_f.as<LevelJob>() + L.job_begin,
+                             e->dtaskoff.as<int32_t>() + L.taskoff_begin,
+                             e->dblkjob.as<int32_t>() + L.blk_begin, L.ntasks, L.chunk, L.grid,
+                             e->dparts.as<double>(), st));
+    for (const Launch& L : e->adj)
+        HIP_TRY(launch_level(true, L.S, e->djobs_a.as<LevelJob>() + L.job_begin,
+                             e->dtaskoff.as<int32_t>() + L.taskoff_begin,
+                             e->dblkjob.as<int32_t>() + L.blk_begin, L.ntasks, L.chunk, L.grid,
+                             e->dparts.as<double>(), st));
+    for (const auto& F : e->fused)
+        HIP_TRY(launch_fused(true, F.S, e->dsched.as<ItemSched>() + F.sched_begin, F.nitems,
+                             F.nblocks, e->waves, e->djobs_f.as<LevelJob>(),
+                             e->dlvl_f.as<int32_t>(), e->djobs_a.as<LevelJob>(),
+                             e->dlvl_a.as<int32_t>(), e->dparts.as<double>(), st));
+    HIP_TRY(launch_reduce(e->dred.as<RedOp>(), (int)e->red.size(), e->dparts.as<double>(),
+                          e->dout.as<double>(), st));
+    HIP_TRY(launch_lf_post(e->dhmc_items.as<HmcItem>(), n, dx, dr, dg, ded, e->dout.as<double>(),
+                           e->dlp.as<double>(), st));
+    HIP_TRY(hipMemcpyAsync(h, dx, 3 * T * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(h + 3 * T + 2 * (size_t)n, e->dlp.p, n * sizeof(double),
+                           hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(h + 3 * T + 3 * (size_t)n, e->dout.p, e->total_out * sizeof(double),
+                           hipMemcpyDeviceToHost, st));
+    return MCP_OK;
+}
+
+static int engine_leapfrog(mcp_engine* e, const double* eps_delta, double* x, double* r, double* g,
+                           double* lp, double* ll) {
+    if (!e || !eps_delta || !x || !r || !g || !lp) return fail(MCP_ERR_INVALID, "null argument");
+    for (auto& it : e->items)
+        if (!it.model_set) return fail(MCP_ERR_STATE, "substitution model not set");
+    DeviceGuard guard(e->device);
+    int st;
+    if ((st = hmc_prepare(e))) return st;
+    const size_t T = e->total_blv;
+    const int n = (int)e->items.size();
+    double* h = e->hhmc.as<double>();
+    std::memcpy(h, x, T * sizeof(double));
+    std::memcpy(h + T, r, T * sizeof(double));
+    std::memcpy(h + 2 * T, g, T * sizeof(double));
+    std::memcpy(h + 3 * T, eps_delta, 2 * (size_t)n * sizeof(double));
+    if (e->use_graph) {
+        if (!e->graph_exec[2]) {
+            HIP_TRY(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+            st = enqueue_leapfrog(e);
+            hipGraph_t cap = nullptr;
+            hipError_t ce = hipStreamEndCapture(e->stream, &cap);
+            if (st) {
+                if (cap) (void)hipGraphDestroy(cap);
+                return st;
+            }
+            if (ce != hipSuccess) return fail(MCP_ERR_HIP, "graph capture failed");
+            hipError_t ie = hipGraphInstantiate(&e->graph_exec[2], cap, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(cap);
+            if (ie != hipSuccess) {
+                e->graph_exec[2] = nullptr;
+                return fail(MCP_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(ie));
+            }
+        }
+        HIP_TRY(hipGraphLaunch(e->graph_exec[2], e->stream));
+    } else {
+        if ((st = enqueue_leapfrog(e))) return st;
+    }
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    std::memcpy(x, h, T * sizeof(double));
+    std::memcpy(r, h + T, T * sizeof(double));
+    std::memcpy(g, h + 2 * T, T * sizeof(double));
+    std::memcpy(lp, h + 3 * T + 2 * (size_t)n, n * sizeof(double));
+    if (ll) {
+        const double* o = h + 3 * T + 3 * (size_t)n;
+        for (int i = 0; i < n; ++i) ll[i] = o[e->items[i].out_off];
+    }
+    return MCP_OK;
+}
+
+// ---------------------------------------------------------------------------
 // C ABI
 // ---------------------------------------------------------------------------
 
@@ -1162,6 +1294,27 @@ int mcp_batch_logl(mcp_batch* b, const double* blv, double* ll) {
 int mcp_batch_logl_grad(mcp_batch* b, const double* blv, double* ll, double* grad) {
     if (!grad) return fail(MCP_ERR_INVALID, "null gradient");
     return engine_eval(b, blv, ll, grad, true);
+}
+
+int mcp_hmc_set_prior(mcp_batch* b, int32_t item, int32_t kind, double scale) {
+    if (!b) return fail(MCP_ERR_INVALID, "null handle");
+    if (item < 0 || item >= (int32_t)b->items.size()) return fail(MCP_ERR_INVALID, "item out of range");
+    if (kind != 0 && kind != 1) return fail(MCP_ERR_INVALID, "unknown prior");
+    if (kind == 1 && !(scale > 0.0)) return fail(MCP_ERR_INVALID, "prior scale must be positive");
+    DeviceGuard guard(b->device);
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    int st = hmc_prepare(b);
+    if (st) return st;
+    b->hmc_items[item].prior = kind;
+    b->hmc_items[item].prior_scale = kind == 1 ? scale : 1.0;
+    HIP_TRY(hipMemcpy(b->dhmc_items.as<HmcItem>() + item, &b->hmc_items[item], sizeof(HmcItem),
+                      hipMemcpyHostToDevice));
+    return MCP_OK;
+}
+
+int mcp_hmc_leapfrog(mcp_batch* b, const double* eps_delta, double* x, double* r, double* g,
+                     double* logpost, double* ll) {
+    return engine_leapfrog(b, eps_delta, x, r, g, logpost, ll);
 }
 
 int mcp_use_graph(mcp_batch* b, int32_t on) {

@@ -83,3 +83,18 @@ struct RedOp {
 };
 
 }  // namespace mcp
+
+namespace mcp {
+
+// Per-item record of the device leapfrog (reference: refraction!,
+// src/samplers/tree_hamiltonian/refraction.jl:2-31, without NNI moves).
+// State buffers are packed per item at offset `off` (n = N-1 branches).
+struct HmcItem {
+    int32_t n;             // branches
+    int32_t off;           // offset of this item's branch vectors
+    int32_t out;           // offset of (ll, grad) in the engine's output buffer
+    int32_t prior;         // 0: none, 1: exponential(scale) per branch
+    double prior_scale;
+};
+
+}  // namespace mcp

@@ -644,3 +644,160 @@ hipError_t launch_reduce(const RedOp* ops, int nops, const double* parts, double
 }
 
 }  // namespace mcp
+
+namespace mcp {
+
+// ---------------------------------------------------------------------------
+// device leapfrog (one block per item)
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ double molifier(double x, double delta) {
+    // auxilliary.jl:117-119
+    return x >= delta ? x : (x * x + delta * delta) / (2.0 * delta);
+}
+
+__device__ __forceinline__ double scale_fac(double x, double delta) {
+    // auxilliary.jl:113-115
+    return x < delta ? x / delta : 1.0;
+}
+
+// Block-wide argmin of v over [0, n) (ties: lowest index); result valid in
+// every thread.
+__device__ __forceinline__ void block_argmin(double v, int idx, double* sv, int* si, double& best,
+                                             int& besti) {
+    // v/idx of threads without an element are +inf / INT_MAX
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const double ov = __shfl_xor(v, off, 64);
+        const int oi = __shfl_xor(idx, off, 64);
+        if (ov < v || (ov == v && oi < idx)) {
+            v = ov;
+            idx = oi;
+        }
+    }
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+    __syncthreads();
+    if (lane == 0) {
+        sv[wid] = v;
+        si[wid] = idx;
+    }
+    __syncthreads();
+    best = sv[0];
+    besti = si[0];
+    for (int w = 1; w < nw; ++w)
+        if (sv[w] < best || (sv[w] == best && si[w] < besti)) {
+            best = sv[w];
+            besti = si[w];
+        }
+}
+
+// First half of a leapfrog for every item:
+//   r += eps/2 g;  y = x + eps r;  reflect at zero in time order
+//   (refraction.jl:35-91 without the NNI attempt);  x = molifier(y, delta);
+//   blv = x (input of the likelihood).
+// The reflection loop is bounded by `max_reflect` rounds; a coordinate still
+// non-positive after that is reflected with abs() (every block exits).
+__global__ void __launch_bounds__(256) k_lf_pre(const HmcItem* __restrict__ items,
+                                                double* __restrict__ x, double* __restrict__ r,
+                                                const double* __restrict__ g,
+                                                const double* __restrict__ eps_delta,
+                                                double* __restrict__ blv, int max_reflect) {
+    __shared__ double sv[4];
+    __shared__ int si[4];
+    __shared__ int s_any;
+    const HmcItem it = items[blockIdx.x];
+    const double eps = eps_delta[2 * blockIdx.x];
+    const double delta = eps_delta[2 * blockIdx.x + 1];
+    double* xi = x + it.off;
+    double* ri = r + it.off;
+    const double* gi = g + it.off;
+    for (int k = threadIdx.x; k < it.n; k += blockDim.x) ri[k] += (eps * 0.5) * gi[k];
+    __syncthreads();
+    // y = x + eps r (kept in x until the end; blv vector of the reflections in blv)
+    double* b = blv + it.off;
+    for (int k = threadIdx.x; k < it.n; k += blockDim.x) b[k] = xi[k];
+    __syncthreads();
+    double t = 0.0;
+    for (int round = 0; round < max_reflect; ++round) {
+        // tmpB = b + (eps - t) r ; stop when all > 0
+        double best = INFINITY;
+        int besti = 0x7fffffff;
+        if (threadIdx.x == 0) s_any = 0;
+        __syncthreads();
+        for (int k = threadIdx.x; k < it.n; k += blockDim.x) {
+            const double tb = b[k] + (eps - t) * ri[k];
+            if (tb <= 0.0) s_any = 1;
+            const double tl = tb / fabs(ri[k]);
+            if (tl < best) {
+                best = tl;
+                besti = k;
+            }
+        }
+        __syncthreads();
+        if (!s_any) break;
+        double tmin;
+        int kmin;
+        block_argmin(best, besti, sv, si, tmin, kmin);
+        // move to the first crossing, reflect that coordinate
+        const double temp = eps - t + tmin;
+        for (int k = threadIdx.x; k < it.n; k += blockDim.x) b[k] = fabs(b[k] + temp * ri[k]);
+        __syncthreads();
+        if (threadIdx.x == 0) ri[kmin] = -ri[kmin];
+        t = eps + tmin;
+        __syncthreads();
+    }
+    for (int k = threadIdx.x; k < it.n; k += blockDim.x) {
+        const double y = fabs(b[k] + (eps - t) * ri[k]);
+        const double xm = molifier(y, delta);
+        xi[k] = xm;
+        b[k] = xm;
+    }
+}
+
+// Second half: g = (dll + dlogprior) * scale_fac(x, delta); r += eps/2 g;
+// lp = ll + logprior(x).
+__global__ void __launch_bounds__(256) k_lf_post(const HmcItem* __restrict__ items,
+                                                 const double* __restrict__ x,
+                                                 double* __restrict__ r, double* __restrict__ g,
+                                                 const double* __restrict__ eps_delta,
+                                                 const double* __restrict__ out,
+                                                 double* __restrict__ lp) {
+    __shared__ double red[16];
+    const HmcItem it = items[blockIdx.x];
+    const double eps = eps_delta[2 * blockIdx.x];
+    const double delta = eps_delta[2 * blockIdx.x + 1];
+    const double* xi = x + it.off;
+    double* ri = r + it.off;
+    double* gi = g + it.off;
+    const double* grad = out + it.out + 1;
+    double lpr = 0.0;
+    for (int k = threadIdx.x; k < it.n; k += blockDim.x) {
+        double gk = grad[k];
+        if (it.prior == 1) {
+            gk += -1.0 / it.prior_scale;
+            lpr += -log(it.prior_scale) - xi[k] / it.prior_scale;
+        }
+        gk *= scale_fac(xi[k], delta);
+        gi[k] = gk;
+        ri[k] += (eps * 0.5) * gk;
+    }
+    const double s = block_sum(lpr, red);
+    if (threadIdx.x == 0) lp[blockIdx.x] = out[it.out] + s;
+}
+
+hipError_t launch_lf_pre(const HmcItem* items, int nitems, double* x, double* r, const double* g,
+                         const double* eps_delta, double* blv, int max_reflect, hipStream_t st) {
+    hipLaunchKernelGGL(k_lf_pre, dim3(nitems), dim3(256), 0, st, items, x, r, g, eps_delta, blv,
+                       max_reflect);
+    return hipGetLastError();
+}
+
+hipError_t launch_lf_post(const HmcItem* items, int nitems, const double* x, double* r, double* g,
+                          const double* eps_delta, const double* out, double* lp,
+                          hipStream_t st) {
+    hipLaunchKernelGGL(k_lf_post, dim3(nitems), dim3(256), 0, st, items, x, r, g, eps_delta, out,
+                       lp);
+    return hipGetLastError();
+}
+
+}  // namespace mcp

@@ -85,6 +85,25 @@ class _Handle:
         lib.mcp_get_launch_times(self._h, dptr(ms), iptr(ph), n)
         return ms[:n], ph[:n]
 
+    def set_prior_exponential(self, item: int, scale: float) -> None:
+        """Exponential(mean=scale) prior on every branch of `item` for the leapfrog."""
+        check(_lib.load_library().mcp_hmc_set_prior(self._h, int(item), 1, float(scale)))
+
+    def leapfrog(self, eps_delta, x, r, g):
+        """One device leapfrog for every item (see mcp_hmc_leapfrog).  x, r, g
+        are updated in place (float64, blv_concat layout); returns
+        (logpost[n_items], ll[n_items])."""
+        ed = np.ascontiguousarray(eps_delta, dtype=np.float64)
+        n = ed.shape[0] // 2
+        lp = np.zeros(n)
+        ll = np.zeros(n)
+        for a in (x, r, g):
+            if a.dtype != np.float64 or not a.flags.c_contiguous:
+                raise ValueError("x, r, g must be contiguous float64 arrays")
+        check(_lib.load_library().mcp_hmc_leapfrog(self._h, dptr(ed), dptr(x), dptr(r), dptr(g),
+                                                   dptr(lp), dptr(ll)))
+        return lp, ll
+
     def algorithmic_bytes(self) -> np.ndarray:
         out = np.zeros(2)
         check(_lib.load_library().mcp_get_bytes(self._h, dptr(out)))

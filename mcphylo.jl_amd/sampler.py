@@ -1,0 +1,371 @@
+"""NUTS over branch lengths with refraction at zero, driven by the device
+leapfrog (mcp_hmc_leapfrog), and the reference's ESS estimator.
+
+The sampler restates the tree sampler of MCPhylo for a fixed topology:
+  nuts_sub! / buildtree (src/samplers/pnuts.jl:179-419): multinomial
+      trajectory sampling with biased progressive sampling at the top level
+      and uniform progressive sampling inside subtrees, divergence check
+      `lu + (logp' - logp0) < 1000`, acceptance statistic of the last
+      doubling for step-size adaptation;
+  refraction! (src/samplers/tree_hamiltonian/refraction.jl:2-91): the
+      leapfrog with time-ordered reflection at zero, molifier and scale_fac
+      (auxilliary.jl:113-119) -- executed on the device, without the NNI
+      topology attempt;
+  nutsepsilon (auxilliary.jl:6-51): initial step size;
+  dual_averaging / setadapt! (pnuts.jl:145-176): step-size adaptation with
+      (gamma, kappa, t0) = (0.05, 0.75, 10) and mu = log(10 eps);
+  ESS = min((SD / MCSE_bm)^2, n), batch size 100 (src/output/stats.jl:167-172,
+      src/output/mcse.jl:26-34).
+Differences, each deliberate: the U-turn test is the Euclidean criterion on
+branch-length vectors (the reference's `proj_euc`, pnuts.jl:454-481, which
+equals its BHV test for a fixed topology); the two trajectory ends integrate
+in directions -1 and +1 (pnuts.jl:197-198 gives both -1); a NaN acceptance
+statistic counts as 0 (the reference adds -Inf).
+
+Chains are Python generators that yield leapfrog requests (x, r, g, eps,
+delta) and receive the new state; `run_nuts` advances any number of chains
+in lockstep, one batched device leapfrog per step, so the chains of a
+ragged batch share every launch.
+"""
+
+from __future__ import annotations
+
+import math
+import time
+
+import numpy as np
+
+__all__ = ["HState", "NUTSChain", "run_nuts", "ess_bm", "mcse_bm", "EngineLeapfrog",
+           "HostLeapfrog"]
+
+
+class HState:
+    """Phase-space point: position x, momentum r, gradient g, log posterior lf."""
+
+    __slots__ = ("x", "r", "g", "lf")
+
+    def __init__(self, x, r, g, lf):
+        self.x, self.r, self.g, self.lf = x, r, g, lf
+
+    def copy(self):
+        return HState(self.x.copy(), self.r.copy(), self.g.copy(), self.lf)
+
+    def neg_energy(self) -> float:
+        """hamiltonian() of the reference: lf - r.r/2 (states.jl:56)."""
+        return self.lf - 0.5 * float(self.r @ self.r)
+
+
+def _logaddexp(a, b):
+    if a == -math.inf:
+        return b
+    if b == -math.inf:
+        return a
+    m = max(a, b)
+    return m + math.log(math.exp(a - m) + math.exp(b - m))
+
+
+def _no_uturn(minus: HState, plus: HState) -> bool:
+    dx = plus.x - minus.x
+    return float(dx @ minus.r) >= 0.0 and float(dx @ plus.r) >= 0.0
+
+
+class _Meta:
+    __slots__ = ("alpha", "nalpha", "leapfrogs")
+
+    def __init__(self):
+        self.alpha = 0.0
+        self.nalpha = 0
+        self.leapfrogs = 0
+
+
+class NUTSChain:
+    """One chain: current position, step size and its adaptation state."""
+
+    def __init__(self, x0, rng, target: float = 0.6, tree_depth: int = 10,
+                 epsilon: float | None = None, jitter: float = 0.0):
+        self.x = np.array(x0, dtype=np.float64)
+        self.rng = rng
+        self.target = float(target)
+        self.tree_depth = int(tree_depth)
+        self.epsilon = epsilon
+        self.jitter = float(jitter)
+        # NUTSstepadapter: m, s_bar, x_bar; params mu, delta=target, gamma, kappa, t0
+        self.m = 0
+        self.s_bar = 0.0
+        self.x_bar = 0.0
+        self.mu = 0.5
+        self.gamma, self.kappa, self.t0 = 0.05, 0.75, 10
+        self.metro_acc_prob = 0.0
+        self.adapting = False
+        self.lf = math.nan
+        self.depths: list[int] = []
+        self.leapfrogs = 0
+
+    # -- requests -----------------------------------------------------------
+    def _step(self, s: HState, eps: float, delta: float):
+        """Yield one leapfrog request; returns the new state."""
+        res = yield (s.x.copy(), s.r.copy(), s.g.copy(), eps, delta)
+        self.leapfrogs += 1
+        return res
+
+    def _find_epsilon(self):
+        """nutsepsilon (auxilliary.jl:6-51) with a cap on the search."""
+        n = self.x.shape[0]
+        log_target = math.log(self.target)
+        zero = np.zeros(n)
+        s = yield from self._step(HState(self.x, self.rng.standard_normal(n), zero, 0.0), 0.0,
+                                  2.0)
+        g0, lf0 = s.g.copy(), s.lf
+        eps = 1.0
+        x1 = HState(s.x.copy(), self.rng.standard_normal(n), g0, lf0)
+        H0 = -x1.neg_energy()
+        s2 = yield from self._step(x1.copy(), eps, 2 * eps)
+        prob = H0 + s2.neg_energy()
+        direction = 1 if prob > log_target else -1
+        for _ in range(60):
+            if not (prob > log_target if direction == 1 else prob < log_target):
+                break
+            eps = 2 * eps if direction == 1 else 0.5 * eps
+            x1.r = self.rng.standard_normal(n)
+            H0 = -x1.neg_energy()
+            s2 = yield from self._step(x1.copy(), eps, 2 * eps)
+            p = H0 + s2.neg_energy()
+            prob = p if math.isfinite(p) else -math.inf
+        return eps
+
+    def _buildtree(self, s: HState, direction: int, j: int, eps: float, delta: float,
+                   logp0: float, lu: float, meta: _Meta):
+        """Subtree of 2^j leapfrogs from s in `direction` (buildtree,
+        pnuts.jl:300-419).  Returns (xprime, far, near, n', s', log_sum_weight):
+        the sample drawn from the subtree, its end away from s, its end next
+        to s."""
+        if j == 0:
+            s1 = yield from self._step(s, direction * eps, delta)
+            logpprime = -s1.neg_energy()
+            lsw = logp0 - logpprime
+            if not math.isfinite(lsw):
+                lsw = -math.inf
+            sprime = (lu + (logpprime - logp0)) < 1000.0
+            a = math.exp(min(0.0, logp0 - logpprime)) if math.isfinite(logpprime) else 0.0
+            meta.alpha += a
+            meta.nalpha += 1
+            nprime = 1 if (logp0 + lu) < logpprime else 0
+            return s1, s1, s1, nprime, bool(sprime), lsw
+        xprime, far, near, n1, s1, lsw_init = yield from self._buildtree(
+            s, direction, j - 1, eps, delta, logp0, lu, meta)
+        if not s1:
+            return xprime, far, near, n1, s1, lsw_init
+        x2, far2, _, n2, s2, lsw_final = yield from self._buildtree(
+            far, direction, j - 1, eps, delta, logp0, lu, meta)
+        ls_final = _logaddexp(lsw_init, lsw_final)
+        if lsw_final > ls_final or (math.isfinite(lsw_final) and
+                                    self.rng.random() < math.exp(lsw_final - ls_final)):
+            xprime = x2
+        ok = _no_uturn(near, far2) if direction == 1 else _no_uturn(far2, near)
+        return xprime, far2, near, n1 + n2, bool(s2 and ok), ls_final
+
+    def iteration(self, adapt: bool):
+        """One NUTS transition (nuts_sub!, pnuts.jl:179-295) plus step-size
+        adaptation (sample!, pnuts.jl:103-143).  Generator."""
+        if self.epsilon is None or not math.isfinite(self.epsilon):
+            self.epsilon = yield from self._find_epsilon()
+        if adapt and not self.adapting:
+            self.m = 0
+            self.mu = math.log(10.0 * self.epsilon)
+        self.adapting = adapt
+        if adapt:
+            self.m += 1
+            eps = self.epsilon
+        else:
+            if self.m > 0:
+                self.epsilon = math.exp(self.x_bar)
+            eps = self.epsilon * (1.0 + self.jitter * (2.0 * self.rng.random() - 1.0))
+        delta = 2.0 * self.epsilon
+        n = self.x.shape[0]
+        r = self.rng.standard_normal(n)
+        s0 = yield from self._step(HState(self.x, r, np.zeros(n), 0.0), 0.0, delta)
+        minus, plus, chosen = s0.copy(), s0.copy(), s0
+        logp0 = -s0.neg_energy()
+        lu = math.log(self.rng.random())
+        log_sum_weight = 0.0
+        meta = _Meta()
+        j = 0
+        while j < self.tree_depth:
+            direction = -1 if self.rng.random() > 0.5 else 1
+            meta.alpha, meta.nalpha = 0.0, 0
+            if direction == -1:
+                xprime, minus, _, nprime, sprime, lsw = yield from self._buildtree(
+                    minus, -1, j, eps, delta, logp0, lu, meta)
+            else:
+                xprime, plus, _, nprime, sprime, lsw = yield from self._buildtree(
+                    plus, 1, j, eps, delta, logp0, lu, meta)
+            self.metro_acc_prob = min(1.0, meta.alpha / meta.nalpha) if meta.nalpha else 0.0
+            if not sprime:
+                break
+            if lsw > log_sum_weight or (math.isfinite(lsw) and
+                                        self.rng.random() < math.exp(lsw - log_sum_weight)):
+                chosen = xprime
+            log_sum_weight = _logaddexp(log_sum_weight, lsw)
+            j += 1
+            if not _no_uturn(minus, plus):
+                break
+        self.x = chosen.x.copy()
+        self.lf = chosen.lf
+        self.depths.append(j)
+        if adapt:
+            # dual_averaging (pnuts.jl:145-159)
+            HT = self.target - self.metro_acc_prob
+            eta = 1.0 / (self.m + self.t0)
+            self.s_bar = (1.0 - eta) * self.s_bar + eta * HT
+            xx = self.mu - self.s_bar * math.sqrt(self.m) / self.gamma
+            xe = self.m ** (-self.kappa)
+            self.x_bar = (1.0 - xe) * self.x_bar + xe * xx
+            self.epsilon = math.exp(xx)
+        return self.x
+
+
+class EngineLeapfrog:
+    """Leapfrog provider backed by a device engine (Plan or Batch): item i is
+    chain i."""
+
+    def __init__(self, engine, n_items: int, n_branches):
+        self.engine = engine
+        self.n_items = n_items
+        self.nb = np.asarray(n_branches)
+        self.off = np.concatenate([[0], np.cumsum(self.nb)])
+        T = int(self.off[-1])
+        self.x = np.zeros(T)
+        self.r = np.zeros(T)
+        self.g = np.zeros(T)
+        self.ed = np.zeros(2 * n_items)
+
+    def __call__(self, reqs):
+        for i, (x, r, g, eps, delta) in enumerate(reqs):
+            a, b = self.off[i], self.off[i + 1]
+            self.x[a:b] = x
+            self.r[a:b] = r
+            self.g[a:b] = g
+            self.ed[2 * i] = eps
+            self.ed[2 * i + 1] = delta
+        lp, _ = self.engine.leapfrog(self.ed, self.x, self.r, self.g)
+        return [HState(self.x[self.off[i]:self.off[i + 1]].copy(),
+                       self.r[self.off[i]:self.off[i + 1]].copy(),
+                       self.g[self.off[i]:self.off[i + 1]].copy(), float(lp[i]))
+                for i in range(self.n_items)]
+
+
+class HostLeapfrog:
+    """Leapfrog provider from a host function logfgrad(x) -> (lf, grad)
+    (the CPU path; also used by the tests with toy targets)."""
+
+    def __init__(self, logfgrad, prior_scale: float | None = None):
+        self.logfgrad = logfgrad
+        self.prior_scale = prior_scale
+
+    def one(self, x, r, g, eps, delta):
+        from_oracle = _host_refraction(x, r, g, eps, delta)
+        xm, r1 = from_oracle
+        lf, gr = self.logfgrad(xm)
+        if self.prior_scale is not None:
+            lf += float(np.sum(-math.log(self.prior_scale) - xm / self.prior_scale))
+            gr = gr - 1.0 / self.prior_scale
+        gr = gr * np.where(xm < delta, xm / delta, 1.0)
+        r1 = r1 + (eps * 0.5) * gr
+        return HState(xm, r1, gr, lf)
+
+    def __call__(self, reqs):
+        return [self.one(*q) for q in reqs]
+
+
+def _host_refraction(x, r, g, eps, delta, max_reflect=None):
+    """First half of refraction! on the host (same steps as the device)."""
+    r = r + (eps * 0.5) * g
+    b = x.copy()
+    t = 0.0
+    cap = 64 * max(1, x.shape[0]) if max_reflect is None else max_reflect
+    for _ in range(cap):
+        tb = b + (eps - t) * r
+        if not np.any(tb <= 0.0):
+            break
+        with np.errstate(divide="ignore", invalid="ignore"):
+            tl = tb / np.abs(r)
+        k = int(np.argmin(tl))
+        temp = eps - t + tl[k]
+        b = np.abs(b + temp * r)
+        r = r.copy()
+        r[k] = -r[k]
+        t = eps + tl[k]
+    y = np.abs(b + (eps - t) * r)
+    xm = np.where(y >= delta, y, (y * y + delta * delta) / (2.0 * delta))
+    return xm, r
+
+
+def run_nuts(chains, leapfrog, n_iter: int, n_adapt: int = 0, trace=None):
+    """Advance every chain through n_iter transitions (the first n_adapt with
+    step-size adaptation) in lockstep: each round issues one batched
+    leapfrog.  Chains that finish early repeat a zero-length step until all
+    are done.  Returns (draws [n_chains][n_iter][n_branches], lf trace,
+    wall seconds, leapfrog rounds)."""
+    n = len(chains)
+    draws = [[] for _ in range(n)]
+    lfs = [[] for _ in range(n)]
+    done_iters = [0] * n
+    gens = [None] * n
+    reqs = [None] * n
+
+    def start(i):
+        gens[i] = chains[i].iteration(done_iters[i] < n_adapt)
+        reqs[i] = next(gens[i])
+
+    for i in range(n):
+        start(i)
+    rounds = 0
+    t0 = time.perf_counter()
+    while any(g is not None for g in gens):
+        live = [q if q is not None else _idle(chains[i]) for i, q in enumerate(reqs)]
+        results = leapfrog(live)
+        rounds += 1
+        for i in range(n):
+            if gens[i] is None:
+                continue
+            try:
+                reqs[i] = gens[i].send(results[i])
+            except StopIteration:
+                if done_iters[i] >= n_adapt:
+                    draws[i].append(chains[i].x.copy())
+                    lfs[i].append(chains[i].lf)
+                done_iters[i] += 1
+                if done_iters[i] < n_iter + n_adapt:
+                    start(i)
+                else:
+                    gens[i] = None
+                    reqs[i] = None
+                if trace is not None:
+                    trace(i, done_iters[i])
+    wall = time.perf_counter() - t0
+    return [np.array(d) for d in draws], [np.array(v) for v in lfs], wall, rounds
+
+
+def _idle(chain):
+    z = np.zeros_like(chain.x)
+    return (chain.x.copy(), z, z, 0.0, 2.0 * (chain.epsilon or 1.0))
+
+
+def mcse_bm(x, size: int = 100) -> float:
+    """Batch-means MCSE (src/output/mcse.jl:26-34): sem of the batch means."""
+    x = np.asarray(x, dtype=np.float64)
+    m = x.shape[0] // size
+    if m < 2:
+        raise ValueError(f"iterations are < {2 * size} and batch size is > {x.shape[0] // 2}")
+    mbar = x[: m * size].reshape(m, size).mean(axis=1)
+    return float(np.std(mbar, ddof=1) / math.sqrt(m))
+
+
+def ess_bm(x, size: int = 100) -> float:
+    """ESS = min((SD / MCSE_bm)^2, n) (src/output/stats.jl:167-172)."""
+    x = np.asarray(x, dtype=np.float64)
+    sd = float(np.std(x, ddof=1))
+    se = mcse_bm(x, size)
+    if se == 0.0:
+        return float(x.shape[0])
+    return float(min((sd / se) ** 2, x.shape[0]))

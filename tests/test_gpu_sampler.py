@@ -1,0 +1,96 @@
+"""GPU: the device leapfrog (mcp_hmc_leapfrog) against its host
+restatement with the CPU oracle's logL + gradient, graph replay, and NUTS on
+the reference's simudata fixture."""
+
+import numpy as np
+import pytest
+
+from conftest import load_fixture, mcp, oracle, rel_err
+
+pytestmark = pytest.mark.gpu
+S = mcp.sampler
+
+
+def _simu_plan(schedule="levels"):
+    t, x = load_fixture("simudata")
+    ta = mcp.tree_arrays(t)
+    U, D, Ui, mu = mcp.JC(np.ones(4) / 4)
+    p = mcp.Plan(ta.parent, x, 1)
+    p.set_model(mcp.Model(U, D, Ui, mu, [1.0], np.ones(4) / 4))
+    p.set_schedule(schedule)
+    return t, x, ta, (U, D, Ui, mu), p
+
+
+def _oracle_logfgrad(ta, x, model):
+    U, D, Ui, mu = model
+
+    def f(blv):
+        return oracle.logl_grad(ta, x, U, D, Ui, mu, [1.0], np.ones(4) / 4, blv)
+    return f
+
+
+@pytest.mark.parametrize("eps", [0.0, 0.01, 0.3])
+def test_device_leapfrog_matches_host(eps):
+    t, x, ta, model, p = _simu_plan()
+    p.set_prior_exponential(0, 0.1)
+    rng = np.random.default_rng(3)
+    blv = mcp.get_branchlength_vector(t)
+    r = rng.standard_normal(blv.shape[0]) * 2.0
+    g = rng.standard_normal(blv.shape[0])
+    delta = 0.02
+    host = S.HostLeapfrog(_oracle_logfgrad(ta, x, model), prior_scale=0.1)
+    ref = host.one(blv.copy(), r.copy(), g.copy(), eps, delta)
+    xd, rd, gd = blv.copy(), r.copy(), g.copy()
+    lp, ll = p.leapfrog(np.array([eps, delta]), xd, rd, gd)
+    assert rel_err(xd, ref.x) < 1e-13
+    assert np.array_equal(np.sign(rd), np.sign(ref.r))
+    assert np.max(np.abs(rd - ref.r)) < 1e-8 * max(1.0, np.max(np.abs(ref.r)))
+    assert np.max(np.abs(gd - ref.g)) < 1e-8 * np.max(np.abs(ref.g))
+    assert rel_err(lp[0], ref.lf) < 1e-10
+
+
+def test_leapfrog_graph_replay_matches_eager():
+    t, x, ta, model, p = _simu_plan()
+    p.set_prior_exponential(0, 0.1)
+    rng = np.random.default_rng(4)
+    blv = mcp.get_branchlength_vector(t)
+    states = [(blv.copy(), rng.standard_normal(17), rng.standard_normal(17)) for _ in range(3)]
+    eager = []
+    for xs, rs, gs in states:
+        a, b, c = xs.copy(), rs.copy(), gs.copy()
+        lp, _ = p.leapfrog(np.array([0.05, 0.1]), a, b, c)
+        eager.append((a, b, c, lp[0]))
+    p.use_graph(True)
+    for _ in range(2):
+        for (xs, rs, gs), (ea, eb, ec, el) in zip(states, eager):
+            a, b, c = xs.copy(), rs.copy(), gs.copy()
+            lp, _ = p.leapfrog(np.array([0.05, 0.1]), a, b, c)
+            assert np.array_equal(a, ea) and np.array_equal(b, eb) and np.array_equal(c, ec)
+            assert lp[0] == el
+    p.use_graph(False)
+
+
+def test_nuts_on_simudata_runs_and_batches_exactly():
+    t, x, ta, model, p = _simu_plan()
+    p.set_prior_exponential(0, 0.1)
+    p.use_graph(True)
+    blv = mcp.get_branchlength_vector(t)
+    lf = S.EngineLeapfrog(p, 1, [17])
+    c = S.NUTSChain(blv, np.random.default_rng(11), target=0.8, tree_depth=6)
+    draws, lfs, wall, rounds = S.run_nuts([c], lf, 40, n_adapt=20)
+    d = draws[0]
+    assert d.shape == (40, 17) and np.all(d > 0) and np.all(np.isfinite(lfs[0]))
+    # two chains in one batch == each chain alone on a plan (deterministic kernels)
+    leaf = ta.leaf_nums
+    b = mcp.Batch([{"x_leaves": x[:, :, leaf - 1], "leaf_nums": leaf}],
+                  [{"dataset": 0, "parent": ta.parent, "n_rates": 1}] * 2)
+    U, D, Ui, mu = model
+    for i in range(2):
+        b.set_model(i, mcp.Model(U, D, Ui, mu, [1.0], np.ones(4) / 4))
+        b.set_prior_exponential(i, 0.1)
+    cs = [S.NUTSChain(blv, np.random.default_rng(s), tree_depth=5) for s in (21, 22)]
+    both, *_ = S.run_nuts(cs, S.EngineLeapfrog(b, 2, [17, 17]), 15, n_adapt=5)
+    for k, s in enumerate((21, 22)):
+        c1 = S.NUTSChain(blv, np.random.default_rng(s), tree_depth=5)
+        solo, *_ = S.run_nuts([c1], lf, 15, n_adapt=5)
+        assert np.array_equal(solo[0], both[k])
