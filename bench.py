@@ -164,6 +164,8 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=5)
     ap.add_argument("--batch-workers", type=int, default=8,
                     help="spawned generator processes (0: generate in this process)")
+    ap.add_argument("--only-batched", action="store_true",
+                    help="run only the ragged-batch section (for profiling it alone)")
     ap.add_argument("--schedule", default="levels", choices=["levels", "fused"],
                     help="launch schedule of the timed evaluations")
     args = ap.parse_args()
@@ -181,6 +183,13 @@ def main():
     if mcp._lib.device_count() < 1:
         log("no HIP device visible")
         sys.exit(1)
+    if args.only_batched:
+        if batch_data is None:
+            log("--only-batched needs --batch-trees > 0")
+            sys.exit(2)
+        res = bench_batched(batch_data, args.batch_steps, 2, 16, args.schedule)
+        print(json.dumps({"batched": res}), flush=True)
+        return
     log(f"building workload {args.workload}")
     w = mcp.synthetic.make_workload(args.workload)
     blv = mcp.get_branchlength_vector(w.tree)
