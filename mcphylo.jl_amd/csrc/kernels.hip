@@ -33,84 +33,10 @@
 #include <hip/hip_runtime.h>
 #include <math.h>
 
+#include "device.h"
 #include "engine.h"
 
 namespace mcp {
-
-// ---------------------------------------------------------------------------
-// helpers
-// ---------------------------------------------------------------------------
-
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
-}
-
-// Sum over the block; the result is valid in thread 0.  `lds` holds one
-// double per wavefront.
-__device__ __forceinline__ double block_sum(double v, double* lds) {
-    v = wave_sum(v);
-    const int lane = threadIdx.x & 63;
-    const int wid = threadIdx.x >> 6;
-    const int nw = (blockDim.x + 63) >> 6;
-    __syncthreads();
-    if (lane == 0) lds[wid] = v;
-    __syncthreads();
-    double s = 0.0;
-    if (threadIdx.x == 0)
-        for (int w = 0; w < nw; ++w) s += lds[w];
-    return s;
-}
-
-// Pointers read out of job records are generic; casting them to the global
-// address space lets the compiler emit global (not flat) loads and stores,
-// which wait on the vector-memory counter only.
-#define MCP_GLOBAL __attribute__((address_space(1)))
-
-template <class T>
-__device__ __forceinline__ const MCP_GLOBAL T* gptr(const T* p) {
-    return (const MCP_GLOBAL T*)p;
-}
-template <class T>
-__device__ __forceinline__ MCP_GLOBAL T* gptr_w(T* p) {
-    return (MCP_GLOBAL T*)p;
-}
-
-typedef double dvec2 __attribute__((ext_vector_type(2)));
-
-template <int S>
-__device__ __forceinline__ void load_row(const MCP_GLOBAL double* __restrict__ p, double (&v)[S]) {
-    if constexpr (S % 2 == 0) {
-        const MCP_GLOBAL dvec2* q = reinterpret_cast<const MCP_GLOBAL dvec2*>(p);
-#pragma unroll
-        for (int i = 0; i < S / 2; ++i) {
-            const dvec2 t = q[i];
-            v[2 * i] = t.x;
-            v[2 * i + 1] = t.y;
-        }
-    } else {
-#pragma unroll
-        for (int i = 0; i < S; ++i) v[i] = p[i];
-    }
-}
-
-template <int S>
-__device__ __forceinline__ void store_row(MCP_GLOBAL double* __restrict__ p, const double (&v)[S]) {
-    if constexpr (S % 2 == 0) {
-        MCP_GLOBAL dvec2* q = reinterpret_cast<MCP_GLOBAL dvec2*>(p);
-#pragma unroll
-        for (int i = 0; i < S / 2; ++i) {
-            dvec2 t;
-            t.x = v[2 * i];
-            t.y = v[2 * i + 1];
-            q[i] = t;
-        }
-    } else {
-#pragma unroll
-        for (int i = 0; i < S; ++i) p[i] = v[i];
-    }
-}
 
 // Locate the job owning task t: task_off[j] <= t < task_off[j+1].
 __device__ __forceinline__ int find_job(const int32_t* __restrict__ task_off, int njobs, int t) {
@@ -593,10 +519,20 @@ int cols_per_thread(int S) {
     }
 }
 
+hipError_t launch_level20(bool adjoint, const LevelJob* jobs, const int32_t* task_off,
+                          const int32_t* blk_job, int total, int chunk, int grid, double* parts,
+                          hipStream_t st);
+
+#ifndef MCP_MFMA20
+#define MCP_MFMA20 1
+#endif
+
 hipError_t launch_level(bool adjoint, int S, const LevelJob* jobs, const int32_t* task_off,
                         const int32_t* blk_job, int total, int chunk, int grid, double* parts,
                         hipStream_t st) {
     if (total <= 0 || grid <= 0) return hipSuccess;
+    if (S == 20 && MCP_MFMA20)
+        return launch_level20(adjoint, jobs, task_off, blk_job, total, chunk, grid, parts, st);
     switch (S) {
 #define X(n, c)                                                                                  \
     case n:                                                                                      \
