@@ -122,6 +122,55 @@ def bench_batched(data, steps, warmup, sample_trees, schedule):
     return out
 
 
+def nuts_run(chain, leapfrog, n_adapt, n_iter):
+    """Adaptation, then n_iter timed NUTS transitions; ESS (batch means,
+    batch size 100) of tree length and log posterior per sampling second."""
+    S = mcp.sampler
+    S.run_nuts([chain], leapfrog, 0, n_adapt=n_adapt)
+    log(f"  adapted: step size {chain.epsilon:.3g}")
+    draws, lfs, wall, rounds = S.run_nuts([chain], leapfrog, n_iter, n_adapt=0)
+    log(f"  {n_iter} iterations in {wall:.1f} s")
+    tl = draws[0].sum(axis=1)
+    ess_tl, ess_lf = S.ess_bm(tl), S.ess_bm(lfs[0])
+    return {"iterations": n_iter, "wall_s": wall, "leapfrogs": rounds,
+            "ess_tree_length": ess_tl, "ess_logpost": ess_lf,
+            "ess_per_s": min(ess_tl, ess_lf) / wall,
+            "mean_tree_length": float(tl.mean()), "sd_tree_length": float(tl.std(ddof=1)),
+            "mean_logpost": float(np.mean(lfs[0])), "step_size": chain.epsilon,
+            "mean_tree_depth": float(np.mean(chain.depths[-n_iter:]))}
+
+
+def bench_nuts(n_adapt, n_iter, cpu: bool):
+    """Plain NUTS on the 20-taxon restriction-site workload: device leapfrog
+    (one HIP graph per step) and, for the baseline, the same sampler with the
+    CPU oracle's logL + gradient."""
+    S = mcp.sampler
+    w = mcp.synthetic.make_workload("restriction20")
+    blv = mcp.get_branchlength_vector(w.tree)
+    plan = mcp.Plan(w.ta.parent, w.x, 1)
+    plan.set_model(mcp.Model(w.U, w.D, w.Uinv, w.mu, w.rates, w.pi))
+    plan.set_prior_exponential(0, 0.1)
+    plan.use_graph(True)
+    out = {"workload": "20-taxon Yule tree, restriction-site model pi=(.5,.5), 1000 simulated "
+                       "sites, R=1, exponential(0.1) branch-length prior, NUTS target 0.8, "
+                       "tree depth 5 (the reference's PNUTS test setting)"}
+    c = S.NUTSChain(blv, np.random.default_rng(20261202), target=0.8, tree_depth=5)
+    out["gpu"] = nuts_run(c, S.EngineLeapfrog(plan, 1, [blv.shape[0]]), n_adapt, n_iter)
+    plan.close()
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+        oracle.set_threads(min(os.cpu_count() or 1, 16))
+
+        def logfgrad(x):
+            return oracle.logl_grad(w.ta, w.x, w.U, w.D, w.Uinv, w.mu, w.rates, w.pi, x)
+        c2 = S.NUTSChain(blv, np.random.default_rng(20261202), target=0.8, tree_depth=5)
+        out["cpu"] = nuts_run(c2, S.HostLeapfrog(logfgrad, prior_scale=0.1), n_adapt, n_iter)
+        out["cpu"]["threads"] = min(os.cpu_count() or 1, 16)
+        out["gpu_over_cpu_ess_per_s"] = out["gpu"]["ess_per_s"] / out["cpu"]["ess_per_s"]
+    return out
+
+
 WORKLOADS = {
     "gtr100": ("100 taxa random topology, t~Exp(0.05), 4-state GTR, 4 gamma cats (alpha 0.5), "
                "10,000 weighted patterns, FP64 logL + d logL/d t"),
@@ -164,6 +213,8 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=5)
     ap.add_argument("--batch-workers", type=int, default=8,
                     help="spawned generator processes (0: generate in this process)")
+    ap.add_argument("--nuts-iters", type=int, default=200,
+                    help="post-adaptation NUTS iterations of the ESS/s section (0 skips it)")
     ap.add_argument("--only-batched", action="store_true",
                     help="run only the ragged-batch section (for profiling it alone)")
     ap.add_argument("--schedule", default="levels", choices=["levels", "fused"],
@@ -295,6 +346,10 @@ def main():
             f"{args.batch_trees} trees, taxa ~ U{{16..512}}, patterns ~ LogU[500, 20000], "
             "restriction/GTR-4 50/50, 4 gamma cats, 4 chains per tree, FP64 logL+grad")
         log(f"batched: {result['batched']['ms_per_step']:.2f} ms/step")
+    if args.nuts_iters > 0:
+        log("NUTS ESS/s section")
+        result["nuts"] = bench_nuts(100, args.nuts_iters, not args.no_cpu)
+        log(f"nuts: gpu {result['nuts']['gpu']['ess_per_s']:.1f} ESS/s")
     if not args.no_cpu:
         log("timing CPU baseline")
         result["cpu_baseline"] = cpu_baseline(mcp, w, blv, args.cpu_seconds)
