@@ -200,6 +200,68 @@ int mcp_get_bytes(mcp_batch* batch, double* bytes2);
  * site-pattern evals/s metric). */
 int mcp_get_sizes(mcp_batch* batch, int64_t* sizes2);
 
+/* ---- learned surrogate of logL (delayed-acceptance first stage) ---------
+ * No counterpart in the reference: MCPhylo's samplers call the exact
+ * logpdf at every leapfrog (src/samplers/pnuts.jl:300-330 via refraction!).
+ * These entry points let a sampler integrate its trajectories on a cheap
+ * surrogate and keep the exact likelihood (mcp_logl / mcp_batch_logl) as the
+ * second-stage accept/reject, so the posterior is unchanged.
+ *
+ * Network over d = n_inputs branches (z = standardised log branch lengths):
+ *   h1 = tanh(W1 z + b1), h2 = tanh(W2 h1 + b2),
+ *   y = w3.h2 + b3 + v.z + q.(z*z),  logL~ = y * y_sd + y_mean,
+ *   z_c = (log x_c - x_mean[c]) / x_sd[c].
+ * Matrix products on BF16 MFMA with FP32 accumulation; FP32 master weights.
+ * Flat FP32 parameter layout (offsets from mcp_mlp_layout, D = d padded to
+ * a multiple of 64, H = hidden): W1 [H][D] row-major | b1 [H] | W2 [H][H] |
+ * b2 [H] | w3 [H] | b3 [64] (b3[0] used) | v [D] | q [D].  Padding entries
+ * are zero and stay zero. */
+typedef struct mcp_mlp mcp_mlp;
+
+/* hidden: multiple of 64 (256 for the standard network); Glorot-uniform
+ * weights from `seed`, zero biases and skip, identity scaling. */
+int mcp_mlp_create(mcp_mlp** out, int32_t n_inputs, int32_t hidden, int32_t device,
+                   uint64_t seed);
+void mcp_mlp_destroy(mcp_mlp* mlp);
+
+/* offsets9 = {W1, b1, W2, b2, w3, b3, v, q, n_params} */
+int mcp_mlp_layout(mcp_mlp* mlp, int64_t* offsets9);
+int mcp_mlp_get_params(mcp_mlp* mlp, float* params);
+int mcp_mlp_set_params(mcp_mlp* mlp, const float* params);
+
+/* Feature (x_mean, x_sd: [d]) and target (y_mean, y_sd) standardisation;
+ * set before mcp_mlp_set_data. */
+int mcp_mlp_set_scaling(mcp_mlp* mlp, const float* x_mean, const float* x_sd, float y_mean,
+                        float y_sd);
+
+/* Upload a training set of n (state, logL) pairs: X [n][d] branch lengths
+ * (row-major, > 0), logl [n].  It stays resident in device memory. */
+int mcp_mlp_set_data(mcp_mlp* mlp, const double* X, const double* logl, int64_t n);
+
+/* Adam (0.9, 0.999, 1e-8) on the mean-squared error (1/2n) sum (y - t)^2 of
+ * standardised targets, minibatches of `batch` rows reshuffled each epoch
+ * (seeded), learning rate decaying exponentially from lr to lr_final over
+ * the epochs.  epoch_loss: [epochs] mean batch loss, or NULL. */
+int mcp_mlp_train(mcp_mlp* mlp, int32_t batch, int32_t epochs, float lr, float lr_final,
+                  uint64_t seed, float* epoch_loss);
+
+/* Gradient of the loss above over the n rows of (X, logl), written in the
+ * flat parameter layout; loss to *loss (may be NULL).  n <= 65536. */
+int mcp_mlp_param_grad(mcp_mlp* mlp, const double* X, const double* logl, int64_t n,
+                       float* grad, float* loss);
+
+/* logL~ [n] and, when grad != NULL, d logL~ / d x [n][d] for the rows of X. */
+int mcp_mlp_predict(mcp_mlp* mlp, const double* X, int64_t n, double* logl, double* grad);
+
+/* mcp_hmc_leapfrog on the surrogate posterior logL~(x) + logprior(x) for
+ * `chains` chains (x, r, g: [chains][d]; eps_delta: [2*chains]); prior:
+ * every branch Exponential(mean prior_scale), none if prior_scale <= 0.
+ * Same refraction, molifier and scale_fac steps as mcp_hmc_leapfrog; one HIP
+ * graph replay per call when mcp_mlp_use_graph is on. */
+int mcp_mlp_leapfrog(mcp_mlp* mlp, int32_t chains, double prior_scale, const double* eps_delta,
+                     double* x, double* r, double* g, double* logpost);
+int mcp_mlp_use_graph(mcp_mlp* mlp, int32_t on);
+
 /* Message of the last failed call on this host thread ("" if none). */
 const char* mcp_last_error(void);
 

@@ -37,6 +37,9 @@ EXPORTED = (
     "mcp_batch_set_topology", "mcp_batch_logl", "mcp_batch_logl_grad", "mcp_use_graph",
     "mcp_set_schedule", "mcp_hmc_set_prior", "mcp_hmc_leapfrog",
     "mcp_set_timing", "mcp_get_timing", "mcp_get_launch_times", "mcp_get_bytes", "mcp_get_sizes",
+    "mcp_mlp_create", "mcp_mlp_destroy", "mcp_mlp_layout", "mcp_mlp_get_params",
+    "mcp_mlp_set_params", "mcp_mlp_set_scaling", "mcp_mlp_set_data", "mcp_mlp_train",
+    "mcp_mlp_param_grad", "mcp_mlp_predict", "mcp_mlp_leapfrog", "mcp_mlp_use_graph",
     "mcp_last_error",
 )
 
@@ -57,6 +60,7 @@ class McpArgumentError(McpError, ValueError):
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int32)
+_fp = ctypes.POINTER(ctypes.c_float)
 
 
 class PlanDesc(ctypes.Structure):
@@ -138,6 +142,21 @@ def load_library():
         "mcp_get_launch_times": (ctypes.c_int, [vp, _dp, _ip, ctypes.c_int32]),
         "mcp_get_bytes": (ctypes.c_int, [vp, _dp]),
         "mcp_get_sizes": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_int64)]),
+        "mcp_mlp_create": (ctypes.c_int, [pp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                          ctypes.c_uint64]),
+        "mcp_mlp_destroy": (None, [vp]),
+        "mcp_mlp_layout": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_int64)]),
+        "mcp_mlp_get_params": (ctypes.c_int, [vp, _fp]),
+        "mcp_mlp_set_params": (ctypes.c_int, [vp, _fp]),
+        "mcp_mlp_set_scaling": (ctypes.c_int, [vp, _fp, _fp, ctypes.c_float, ctypes.c_float]),
+        "mcp_mlp_set_data": (ctypes.c_int, [vp, _dp, _dp, ctypes.c_int64]),
+        "mcp_mlp_train": (ctypes.c_int, [vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_float,
+                                         ctypes.c_float, ctypes.c_uint64, _fp]),
+        "mcp_mlp_param_grad": (ctypes.c_int, [vp, _dp, _dp, ctypes.c_int64, _fp, _fp]),
+        "mcp_mlp_predict": (ctypes.c_int, [vp, _dp, ctypes.c_int64, _dp, _dp]),
+        "mcp_mlp_leapfrog": (ctypes.c_int, [vp, ctypes.c_int32, ctypes.c_double, _dp, _dp, _dp,
+                                            _dp, _dp]),
+        "mcp_mlp_use_graph": (ctypes.c_int, [vp, ctypes.c_int32]),
         "mcp_last_error": (ctypes.c_char_p, []),
     }
     for name, (res, args) in sig.items():
@@ -165,6 +184,10 @@ def dptr(a: np.ndarray | None):
 
 def iptr(a: np.ndarray):
     return a.ctypes.data_as(_ip)
+
+
+def fptr(a: np.ndarray | None):
+    return None if a is None else a.ctypes.data_as(_fp)
 
 
 def device_count() -> int:

@@ -61,6 +61,15 @@ int fail(int code, const std::string& msg) {
     return code;
 }
 
+}  // namespace
+
+namespace mcp {
+// error reporting for the other translation units of the library
+int set_last_error(int code, const char* msg) { return fail(code, msg); }
+}  // namespace mcp
+
+namespace {
+
 #define HIP_TRY(expr)                                                                       \
     do {                                                                                    \
         hipError_t e_ = (expr);                                                             \

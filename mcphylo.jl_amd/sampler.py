@@ -36,7 +36,7 @@ import time
 import numpy as np
 
 __all__ = ["HState", "NUTSChain", "run_nuts", "ess_bm", "mcse_bm", "EngineLeapfrog",
-           "HostLeapfrog"]
+           "HostLeapfrog", "SurrogateLeapfrog", "run_da_nuts", "exponential_logprior"]
 
 
 class HState:
@@ -82,7 +82,13 @@ class NUTSChain:
     """One chain: current position, step size and its adaptation state."""
 
     def __init__(self, x0, rng, target: float = 0.6, tree_depth: int = 10,
-                 epsilon: float | None = None, jitter: float = 0.0):
+                 epsilon: float | None = None, jitter: float = 0.0, molifier: bool = True):
+        """molifier=True keeps the reference's smoothing of positions below
+        delta = 2 eps (auxilliary.jl:113-119); its zero-length step at the
+        start of every transition moves such positions, so the transition
+        is not reversible there.  molifier=False sets delta to 1e-300: pure
+        reflection at zero, an exact sampler of the posterior."""
+        self.molifier = bool(molifier)
         self.x = np.array(x0, dtype=np.float64)
         self.rng = rng
         self.target = float(target)
@@ -104,6 +110,8 @@ class NUTSChain:
     # -- requests -----------------------------------------------------------
     def _step(self, s: HState, eps: float, delta: float):
         """Yield one leapfrog request; returns the new state."""
+        if not self.molifier:
+            delta = 1e-300
         res = yield (s.x.copy(), s.r.copy(), s.g.copy(), eps, delta)
         self.leapfrogs += 1
         return res
@@ -226,9 +234,10 @@ class NUTSChain:
 
 class EngineLeapfrog:
     """Leapfrog provider backed by a device engine (Plan or Batch): item i is
-    chain i."""
+    chain i.  With `record` a list, every step of nonzero length appends
+    (chain, x, logL(x)) -- the (state, logL) pairs a surrogate trains on."""
 
-    def __init__(self, engine, n_items: int, n_branches):
+    def __init__(self, engine, n_items: int, n_branches, record=None):
         self.engine = engine
         self.n_items = n_items
         self.nb = np.asarray(n_branches)
@@ -238,6 +247,7 @@ class EngineLeapfrog:
         self.r = np.zeros(T)
         self.g = np.zeros(T)
         self.ed = np.zeros(2 * n_items)
+        self.record = record
 
     def __call__(self, reqs):
         for i, (x, r, g, eps, delta) in enumerate(reqs):
@@ -247,7 +257,12 @@ class EngineLeapfrog:
             self.g[a:b] = g
             self.ed[2 * i] = eps
             self.ed[2 * i + 1] = delta
-        lp, _ = self.engine.leapfrog(self.ed, self.x, self.r, self.g)
+        lp, ll = self.engine.leapfrog(self.ed, self.x, self.r, self.g)
+        if self.record is not None:
+            for i in range(self.n_items):
+                if self.ed[2 * i] != 0.0:
+                    self.record.append((i, self.x[self.off[i]:self.off[i + 1]].copy(),
+                                        float(ll[i])))
         return [HState(self.x[self.off[i]:self.off[i + 1]].copy(),
                        self.r[self.off[i]:self.off[i + 1]].copy(),
                        self.g[self.off[i]:self.off[i + 1]].copy(), float(lp[i]))
@@ -275,6 +290,32 @@ class HostLeapfrog:
 
     def __call__(self, reqs):
         return [self.one(*q) for q in reqs]
+
+
+class SurrogateLeapfrog:
+    """Leapfrog provider on the surrogate posterior logL~(x) + logprior(x)
+    (mcp_mlp_leapfrog): chain i is row i."""
+
+    def __init__(self, surrogate, n_chains: int, prior_scale: float | None = None):
+        self.s = surrogate
+        self.n = int(n_chains)
+        self.prior_scale = prior_scale
+        d = surrogate.d
+        self.x = np.zeros((self.n, d))
+        self.r = np.zeros((self.n, d))
+        self.g = np.zeros((self.n, d))
+        self.ed = np.zeros(2 * self.n)
+
+    def __call__(self, reqs):
+        for i, (x, r, g, eps, delta) in enumerate(reqs):
+            self.x[i] = x
+            self.r[i] = r
+            self.g[i] = g
+            self.ed[2 * i] = eps
+            self.ed[2 * i + 1] = delta
+        lp = self.s.leapfrog(self.ed, self.x, self.r, self.g, self.prior_scale)
+        return [HState(self.x[i].copy(), self.r[i].copy(), self.g[i].copy(), float(lp[i]))
+                for i in range(self.n)]
 
 
 def _host_refraction(x, r, g, eps, delta, max_reflect=None):
@@ -344,6 +385,113 @@ def run_nuts(chains, leapfrog, n_iter: int, n_adapt: int = 0, trace=None):
                     trace(i, done_iters[i])
     wall = time.perf_counter() - t0
     return [np.array(d) for d in draws], [np.array(v) for v in lfs], wall, rounds
+
+
+def _lockstep_iteration(chains, leapfrog, adapt: bool) -> int:
+    """One NUTS transition of every chain, the chains' leapfrogs batched into
+    shared calls; returns the number of calls."""
+    gens = [c.iteration(adapt) for c in chains]
+    reqs = [next(g) for g in gens]
+    rounds = 0
+    while any(g is not None for g in gens):
+        live = [q if q is not None else _idle(chains[i]) for i, q in enumerate(reqs)]
+        results = leapfrog(live)
+        rounds += 1
+        for i, g in enumerate(gens):
+            if g is None:
+                continue
+            try:
+                reqs[i] = g.send(results[i])
+            except StopIteration:
+                gens[i] = None
+                reqs[i] = None
+    return rounds
+
+
+def exponential_logprior(X, scale: float) -> np.ndarray:
+    """Sum over branches of the Exponential(mean scale) log density, per row
+    (exponentialBL, src/Likelihood/Prior.jl:50-56,80-83)."""
+    X = np.atleast_2d(X)
+    return np.sum(-math.log(scale) - X / scale, axis=1)
+
+
+def run_da_nuts(chains, leapfrog, exact_logpost, n_iter: int, n_adapt: int = 0,
+                min_accept: float = 0.0, window: int = 50, fallback=None, trace=None,
+                record=None):
+    """Delayed-acceptance NUTS.  Stage one: a NUTS transition of every chain
+    integrated on the surrogate posterior by `leapfrog` (a SurrogateLeapfrog);
+    its draw x' is a proposal.  Stage two: one batched exact evaluation
+    `exact_logpost(X) -> [n_chains]` and the Metropolis-Hastings correction
+        log a = (lp(x') - lp~(x')) - (lp(x) - lp~(x)),
+    which leaves the exact posterior invariant because the NUTS transition is
+    reversible with respect to the surrogate posterior.  lp~ of the current
+    state is the value the surrogate leapfrog returned for it (the first one
+    from a zero-length step with the molifier off), so both stages compare
+    values from the same kernels.
+
+    Fallback: after adaptation, when the second-stage acceptance over the
+    last `window` iterations (all chains) drops below `min_accept`, the
+    remaining iterations run plain NUTS with `fallback` (an exact leapfrog
+    provider).  With `record` a list, every second-stage evaluation appends
+    (chain, x', exact lp(x')) -- the pairs that refine the surrogate where it
+    proposes.  Returns (draws, exact lp of the draws, wall seconds, stats)."""
+    n = len(chains)
+    d = chains[0].x.shape[0]
+    t0 = time.perf_counter()
+    zero = np.zeros(d)
+    first = leapfrog([(c.x.copy(), zero, zero, 0.0, 1e-300) for c in chains])
+    sur = np.array([s.lf for s in first])
+    lp = np.asarray(exact_logpost(np.stack([c.x for c in chains])), dtype=np.float64).copy()
+    draws = [[] for _ in range(n)]
+    lps = [[] for _ in range(n)]
+    stats = {"rounds": 1, "exact_evals": n, "proposals": 0, "accepted": 0,
+             "fallback_at": None, "accept_trace": []}
+    recent: list[float] = []
+    total = n_adapt + n_iter
+    it = 0
+    while it < total:
+        adapt = it < n_adapt
+        x_old = [c.x.copy() for c in chains]
+        stats["rounds"] += _lockstep_iteration(chains, leapfrog, adapt)
+        X = np.stack([c.x for c in chains])
+        lp_new = np.asarray(exact_logpost(X), dtype=np.float64)
+        stats["exact_evals"] += n
+        if record is not None:
+            record.extend((i, X[i].copy(), float(lp_new[i])) for i in range(n))
+        acc = 0
+        for i, c in enumerate(chains):
+            la = (lp_new[i] - c.lf) - (lp[i] - sur[i])
+            if la >= 0.0 or (math.isfinite(la) and c.rng.random() < math.exp(la)):
+                lp[i] = lp_new[i]
+                sur[i] = c.lf
+                acc += 1
+            else:
+                c.x = x_old[i]
+                c.lf = sur[i]
+        stats["proposals"] += n
+        stats["accepted"] += acc
+        stats["accept_trace"].append(acc / n)
+        if not adapt:
+            for i, c in enumerate(chains):
+                draws[i].append(c.x.copy())
+                lps[i].append(lp[i])
+            recent.append(acc / n)
+            recent = recent[-window:]
+        it += 1
+        if trace is not None:
+            trace(it)
+        if (fallback is not None and not adapt and len(recent) == window
+                and float(np.mean(recent)) < min_accept):
+            stats["fallback_at"] = it
+            rest, rest_lf, _, rounds = run_nuts(chains, fallback, total - it, 0)
+            stats["rounds"] += rounds
+            for i in range(n):
+                draws[i].extend(list(rest[i]))
+                lps[i].extend(list(rest_lf[i]))
+            break
+    wall = time.perf_counter() - t0
+    stats["accept_rate"] = stats["accepted"] / max(1, stats["proposals"])
+    return [np.array(v) for v in draws], [np.array(v) for v in lps], wall, stats
 
 
 def _idle(chain):

@@ -32,6 +32,7 @@ def _load_oracle():
 
 mcp = _load_package()
 oracle = _load_oracle()
+import mlp_oracle  # noqa: E402  (oracle/ is on sys.path after _load_oracle)
 
 
 def pytest_configure(config):
