@@ -171,6 +171,70 @@ def bench_nuts(n_adapt, n_iter, cpu: bool):
     return out
 
 
+def bench_dense(n_sites, steps, cpu: bool, cpu_seconds: float):
+    """The dense 20-state workload (1000 taxa, n_sites sites as leaf state
+    codes, 4 gamma categories) on the FP64-MFMA level kernels; the CPU oracle
+    is timed on a 1000-site slice (the full size needs > 300 GB host memory
+    in the oracle's layout)."""
+    t0 = time.perf_counter()
+    w = mcp.synthetic.make_dense_workload(C=n_sites)
+    log(f"  dense workload generated in {time.perf_counter() - t0:.1f} s")
+    b = mcp.Batch([w.dataset()], [{"dataset": 0, "parent": w.parent, "n_rates": w.R}])
+    b.set_model(0, mcp.Model(w.U, w.D, w.Uinv, w.mu, w.rates, w.pi))
+    b.set_schedule("levels")
+    b.logl_grad(w.blv)
+    t1 = time.perf_counter()
+    for _ in range(steps):
+        b.logl_grad(w.blv)
+    wall = time.perf_counter() - t1
+    b.set_timing(True)
+    b.logl_grad(w.blv)
+    ms, ph = b.launch_times()
+    b.set_timing(False)
+    bytes_fwd, bytes_adj = b.algorithmic_bytes()
+    b.close()
+    fwd_ms, adj_ms = float(np.sum(ms[ph == 2])), float(np.sum(ms[ph == 3]))
+    n = len(w.leaf_nums)
+    # useful FP64 flops: 2 S^2 per (pattern, category) and internal child edge
+    # in the post-order pass, ~8 S^2 per internal and 4 S^2 per leaf branch in
+    # the pre-order pass
+    fl_fwd = 2.0 * w.S ** 2 * w.R * w.C * (2 * (n - 1) - n)
+    fl_adj = 1.0 * w.S ** 2 * w.R * w.C * (8 * (n - 2) + 4 * n)
+    out = {"workload": f"{n} taxa (random coalescence), t ~ Exp(0.05), 20 states "
+                       "(exchangeabilities iid Exp(1), pi ~ Dir(10)), 4 gamma cats (alpha 0.5), "
+                       f"{w.C} simulated sites as uint8 tip codes, FP64 logL + full gradient",
+           "sites": w.C, "nodes": w.N, "steps": steps,
+           "ms_per_step": 1e3 * wall / steps, "value": w.C * steps / wall,
+           "unit": "site-pattern evals/s",
+           "post_order_ms": fwd_ms, "pre_order_ms": adj_ms,
+           "post_order_GBs": bytes_fwd / (fwd_ms * 1e-3) / 1e9,
+           "pre_order_GBs": bytes_adj / (adj_ms * 1e-3) / 1e9,
+           "algorithmic_GB_per_step": (bytes_fwd + bytes_adj) / 1e9,
+           "post_order_fp64_TFLOPs": fl_fwd / (fwd_ms * 1e-3) / 1e12,
+           "pre_order_fp64_TFLOPs": fl_adj / (adj_ms * 1e-3) / 1e12,
+           "fp64_mfma_peak_TFLOPs": 78.6}
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+        threads = min(os.cpu_count() or 1, 16)
+        oracle.set_threads(threads)
+        cols = slice(0, 1000)
+        x = w.tip_partials(cols)
+        ta = mcp.tree_arrays(mcp.tree_from_parent(w.parent, w.blv))
+        k, t2 = 0, time.perf_counter()
+        while True:
+            oracle.logl_grad(ta, x, w.U, w.D, w.Uinv, w.mu, w.rates, w.pi, w.blv)
+            k += 1
+            dt = time.perf_counter() - t2
+            if dt >= cpu_seconds:
+                break
+        out["cpu_baseline"] = {"value": 1000 * k / dt, "unit": "site-pattern evals/s",
+                               "cores": threads, "kind": "port",
+                               "sample": f"{k} logL+grad evaluations of the first 1000 sites "
+                                         f"with the C oracle, {threads} OpenMP threads"}
+    return out
+
+
 WORKLOADS = {
     "gtr100": ("100 taxa random topology, t~Exp(0.05), 4-state GTR, 4 gamma cats (alpha 0.5), "
                "10,000 weighted patterns, FP64 logL + d logL/d t"),
@@ -219,6 +283,9 @@ def main():
                     help="run only the ragged-batch section (for profiling it alone)")
     ap.add_argument("--schedule", default="levels", choices=["levels", "fused"],
                     help="launch schedule of the timed evaluations")
+    ap.add_argument("--dense-sites", type=int, default=100_000,
+                    help="sites of the dense 20-state section (1000 taxa); 0 skips it")
+    ap.add_argument("--dense-steps", type=int, default=5)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -350,6 +417,11 @@ def main():
         log("NUTS ESS/s section")
         result["nuts"] = bench_nuts(100, args.nuts_iters, not args.no_cpu)
         log(f"nuts: gpu {result['nuts']['gpu']['ess_per_s']:.1f} ESS/s")
+    if args.dense_sites > 0:
+        log("dense 20-state section")
+        result["dense20"] = bench_dense(args.dense_sites, args.dense_steps, not args.no_cpu,
+                                        min(args.cpu_seconds, 8.0))
+        log(f"dense20: {result['dense20']['ms_per_step']:.1f} ms/step")
     if not args.no_cpu:
         log("timing CPU baseline")
         result["cpu_baseline"] = cpu_baseline(mcp, w, blv, args.cpu_seconds)

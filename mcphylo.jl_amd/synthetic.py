@@ -280,7 +280,7 @@ def simulate_codes(parent, blv, U, D, Uinv, mu, rates, pi, C: int, rng) -> np.nd
     while k < len(order):
         order.extend(kids[order[k]])
         k += 1
-    state = np.zeros((N, C), dtype=np.int64)
+    state = np.zeros((N, C), dtype=np.uint8 if S <= 255 else np.int64)
     cdf_pi = np.cumsum(pi)
     state[N - 1] = np.minimum(np.searchsorted(cdf_pi, rng.random(C) * cdf_pi[-1]), S - 1)
     for v in order[1:]:
@@ -315,6 +315,50 @@ def _batched_tree(i: int, seed: int, n_range, c_range, R: int, chains: int):
     model = (U, D, Ui, mu, rates, pi)
     blvs = [blv * np.exp(rng.normal(0.0, 0.1, blv.shape)) for _ in range(chains)]
     return ds, parent, model, blvs
+
+
+class DenseWorkload:
+    """A large many-state workload held as leaf state codes (the one-hot tip
+    partials of 1000 taxa x 100,000 sites x 20 states would take 32 GB)."""
+
+    def __init__(self, parent, blv, codes, leaf_nums, table, model):
+        self.parent, self.blv, self.codes, self.leaf_nums = parent, blv, codes, leaf_nums
+        self.table = table
+        self.U, self.D, self.Uinv, self.mu, self.rates, self.pi = model
+        self.S = table.shape[1]
+        self.C = codes.shape[1]
+        self.N = parent.shape[0]
+        self.R = len(self.rates)
+
+    def dataset(self, cols=None):
+        """Batch dataset dict over all sites or the site slice `cols`."""
+        codes = self.codes if cols is None else np.ascontiguousarray(self.codes[:, cols])
+        return {"leaf_nums": self.leaf_nums, "tip_codes": codes, "code_table": self.table,
+                "weights": None}
+
+    def tip_partials(self, cols):
+        """One-hot (S, C, N) tip partials of the site slice (for the oracle)."""
+        codes = self.codes[:, cols]
+        x = np.zeros((self.S, codes.shape[1], self.N), order="F")
+        for k, v in enumerate(self.leaf_nums):
+            x[:, :, v - 1] = self.table[codes[k]].T
+        return x
+
+
+def make_dense_workload(n: int = 1000, C: int = 100_000, S: int = 20, R: int = 4,
+                        seed: int = BASE_SEED) -> DenseWorkload:
+    """n taxa (random pairwise coalescence), t ~ Exp(0.05), S-state model with
+    exchangeabilities iid Exp(1) and pi ~ Dir(10), R gamma categories (alpha
+    0.5), C sites simulated down the tree as leaf state codes."""
+    rng = np.random.default_rng(seed)
+    parent = coalescent_parent(n, rng)
+    blv = rng.exponential(0.05, 2 * n - 2)
+    pi, r = random_gtr(S, rng, 1.0, 10.0, exch="exp")
+    U, D, Ui, mu = GTR(pi, r)
+    rates = discrete_gamma_rates(0.5, 0.5, R) if R > 1 else np.array([1.0])
+    codes, leaf_nums = simulate_codes(parent, blv, U, D, Ui, mu, rates, pi, C, rng)
+    table = np.vstack([np.eye(S), np.ones((1, S))])
+    return DenseWorkload(parent, blv, codes, leaf_nums, table, (U, D, Ui, mu, rates, pi))
 
 
 def make_batched_workload(n_trees: int = 256, chains: int = 4, seed: int = BASE_SEED,
