@@ -235,20 +235,28 @@ int mcp_mlp_set_scaling(mcp_mlp* mlp, const float* x_mean, const float* x_sd, fl
                         float y_sd);
 
 /* Upload a training set of n (state, logL) pairs: X [n][d] branch lengths
- * (row-major, > 0), logl [n].  It stays resident in device memory. */
-int mcp_mlp_set_data(mcp_mlp* mlp, const double* X, const double* logl, int64_t n);
+ * (row-major, > 0), logl [n], and optionally the exact gradients
+ * grad [n][d] = d logL / d x (NULL: none).  It stays resident in device
+ * memory. */
+int mcp_mlp_set_data(mcp_mlp* mlp, const double* X, const double* logl, const double* grad,
+                     int64_t n);
 
-/* Adam (0.9, 0.999, 1e-8) on the mean-squared error (1/2n) sum (y - t)^2 of
- * standardised targets, minibatches of `batch` rows reshuffled each epoch
- * (seeded), learning rate decaying exponentially from lr to lr_final over
- * the epochs.  epoch_loss: [epochs] mean batch loss, or NULL. */
+/* Adam (0.9, 0.999, 1e-8) on the loss
+ *   (1/2n) sum_r (y_r - t_r)^2 + (grad_weight / 2nd) sum_r |dy/dz_r - G_r|^2
+ * of standardised targets t and standardised exact gradients
+ * G = dlogL/dx * x * x_sd / y_sd (gradient matching; grad_weight 0 turns it
+ * off, > 0 needs gradients in mcp_mlp_set_data).  Minibatches of `batch`
+ * rows reshuffled each epoch (seeded); learning rate decaying exponentially
+ * from lr to lr_final over the epochs.  epoch_loss: [epochs] mean batch
+ * loss, or NULL. */
 int mcp_mlp_train(mcp_mlp* mlp, int32_t batch, int32_t epochs, float lr, float lr_final,
-                  uint64_t seed, float* epoch_loss);
+                  float grad_weight, uint64_t seed, float* epoch_loss);
 
-/* Gradient of the loss above over the n rows of (X, logl), written in the
- * flat parameter layout; loss to *loss (may be NULL).  n <= 65536. */
-int mcp_mlp_param_grad(mcp_mlp* mlp, const double* X, const double* logl, int64_t n,
-                       float* grad, float* loss);
+/* Gradient of the loss above over the n rows of (X, logl, dlogl), written in
+ * the flat parameter layout; loss to *loss (may be NULL).  n <= 65536;
+ * dlogl may be NULL when grad_weight is 0. */
+int mcp_mlp_param_grad(mcp_mlp* mlp, const double* X, const double* logl, const double* dlogl,
+                       int64_t n, float grad_weight, float* grad, float* loss);
 
 /* logL~ [n] and, when grad != NULL, d logL~ / d x [n][d] for the rows of X. */
 int mcp_mlp_predict(mcp_mlp* mlp, const double* X, int64_t n, double* logl, double* grad);

@@ -149,10 +149,11 @@ def load_library():
         "mcp_mlp_get_params": (ctypes.c_int, [vp, _fp]),
         "mcp_mlp_set_params": (ctypes.c_int, [vp, _fp]),
         "mcp_mlp_set_scaling": (ctypes.c_int, [vp, _fp, _fp, ctypes.c_float, ctypes.c_float]),
-        "mcp_mlp_set_data": (ctypes.c_int, [vp, _dp, _dp, ctypes.c_int64]),
+        "mcp_mlp_set_data": (ctypes.c_int, [vp, _dp, _dp, _dp, ctypes.c_int64]),
         "mcp_mlp_train": (ctypes.c_int, [vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_float,
-                                         ctypes.c_float, ctypes.c_uint64, _fp]),
-        "mcp_mlp_param_grad": (ctypes.c_int, [vp, _dp, _dp, ctypes.c_int64, _fp, _fp]),
+                                         ctypes.c_float, ctypes.c_float, ctypes.c_uint64, _fp]),
+        "mcp_mlp_param_grad": (ctypes.c_int, [vp, _dp, _dp, _dp, ctypes.c_int64, ctypes.c_float,
+                                              _fp, _fp]),
         "mcp_mlp_predict": (ctypes.c_int, [vp, _dp, ctypes.c_int64, _dp, _dp]),
         "mcp_mlp_leapfrog": (ctypes.c_int, [vp, ctypes.c_int32, ctypes.c_double, _dp, _dp, _dp,
                                             _dp, _dp]),

@@ -51,13 +51,13 @@ constexpr int kTile = 64;   // block tile (M and N)
 constexpr int kKStep = 32;  // K per MFMA step
 constexpr int kLdsK = 40;   // LDS row length in bf16 (32 + 8 padding)
 
-enum { EPI_STORE = 0, EPI_BIAS_TANH = 1, EPI_MUL_DTANH = 2 };
+enum { EPI_STORE = 0, EPI_BIAS_TANH = 1, EPI_MUL_DTANH = 2, EPI_ACCUM = 3 };
 
 // C = epilogue(opA . opB), opA: M x K, opB: K x N, all FP32 in memory.
 //   opA[m][k] = TA ? A[k*lda + m] : A[m*lda + k]
 //   opB[k][n] = TB ? B[n*ldb + k] : B[k*ldb + n]
 // EPI_STORE: C = acc; EPI_BIAS_TANH: C = tanh(acc + aux[n]);
-// EPI_MUL_DTANH: C = acc * (1 - aux[m*ldc + n]^2).
+// EPI_MUL_DTANH: C = acc * (1 - aux[m*ldc + n]^2); EPI_ACCUM: C += acc.
 // Grid (N/64, M/64), 256 threads: four waves, each a 32 x 32 quarter of the
 // tile as 2 x 2 MFMA 16 x 16 blocks.
 template <bool TA, bool TB, int EPI>
@@ -123,6 +123,7 @@ __global__ void __launch_bounds__(256) k_gemm_bf16(const float* __restrict__ A, 
                     const float h = aux[o];
                     v *= 1.f - h * h;
                 }
+                if (EPI == EPI_ACCUM) v += C[o];
                 C[o] = v;
             }
 }
@@ -278,6 +279,109 @@ __global__ void k_finalize(const float* __restrict__ y, const float* __restrict_
     }
 }
 
+// ---- gradient-matching (Sobolev) term of the training loss -----------------
+// Loss  (1/2n) sum_r (y_r - t_r)^2 + (lambda / 2nd) sum_{r, c<d} (g_rc - G_rc)^2
+// with g = dy/dz = W1^T delta1 + v + 2 q z (the input gradient, delta2 =
+// w3 (1 - h2^2), delta1 = (W2^T delta2)(1 - h1^2)) and G the standardised
+// exact gradient.  Its parameter gradient is a second backward pass through
+// the input-gradient graph; the kernels below are its elementwise steps.
+
+// rho[r][c] = lam_n (dZ[r][c] + v[c] + 2 q[c] z[r][c] - G[r][c]) for r < n,
+// c < d (zero elsewhere), lam_n = lambda / (n d); lrow[r] += lam_n/2 sum_c
+// (...)^2.  One wavefront per row.
+__global__ void __launch_bounds__(256) k_sob_rho(const float* __restrict__ dZ,
+                                                 const float* __restrict__ Z,
+                                                 const float* __restrict__ G,
+                                                 const float* __restrict__ v,
+                                                 const float* __restrict__ q,
+                                                 float* __restrict__ rho, float* __restrict__ lrow,
+                                                 int D, int d, int n, int rows, float lam_n) {
+    const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (row >= rows) return;
+    float acc = 0.f;
+    for (int c = lane; c < D; c += 64) {
+        const size_t o = (size_t)row * D + c;
+        float e = 0.f;
+        if (row < n && c < d) e = dZ[o] + v[c] + 2.f * q[c] * Z[o] - G[o];
+        rho[o] = lam_n * e;
+        acc += e * e;
+    }
+    acc = wave_sum_f(acc);
+    if (lane == 0) lrow[row] += 0.5f * lam_n * acc;
+}
+
+// Dl1 = T1 (1 - h1^2)  (delta1 from T1 = delta2 W2)
+__global__ void k_sob_delta1(const float* __restrict__ T1, const float* __restrict__ H1,
+                             float* __restrict__ Dl1, long long total) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    const float h = H1[i];
+    Dl1[i] = T1[i] * (1.f - h * h);
+}
+
+// from kappa1 = rho W1^T (in K1):  K1 <- mu1 = kappa1 (1 - h1^2);
+// Gam1 = -2 h1 kappa1 T1
+__global__ void k_sob_h1(float* __restrict__ K1, const float* __restrict__ T1,
+                         const float* __restrict__ H1, float* __restrict__ Gam1, long long total) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    const float h = H1[i], k1 = K1[i];
+    K1[i] = k1 * (1.f - h * h);
+    Gam1[i] = -2.f * h * k1 * T1[i];
+}
+
+// from kappa2 = mu1 W2^T (in K2), per row r with dy[r]:
+//   A2tot = dy delta2 - 2 h2 kappa2 w3 (1 - h2^2)      -> dA2
+//   K2   <- dy h2 + kappa2 (1 - h2^2)                  (rows of the w3 gradient)
+__global__ void k_sob_h2(float* __restrict__ K2, const float* __restrict__ H2,
+                         const float* __restrict__ D2, const float* __restrict__ w3,
+                         const float* __restrict__ dy, float* __restrict__ dA2, int H,
+                         long long total) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    const int c = (int)(i % H);
+    const float g = dy[i / H];
+    const float h = H2[i], k2 = K2[i], s2 = 1.f - h * h;
+    dA2[i] = g * D2[i] - 2.f * h * k2 * w3[c] * s2;
+    K2[i] = g * h + k2 * s2;
+}
+
+// A1 <- (A1 + Gam1) (1 - h1^2)   (A1 = A2tot W2 on entry)
+__global__ void k_sob_a1(float* __restrict__ A1, const float* __restrict__ Gam1,
+                         const float* __restrict__ H1, long long total) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    const float h = H1[i];
+    A1[i] = (A1[i] + Gam1[i]) * (1.f - h * h);
+}
+
+// gv[c] = sum_r dy_r z + rho;  gq[c] = sum_r dy_r z^2 + 2 z rho  (fixed order)
+__global__ void k_colsum_vq(const float* __restrict__ Z, const float* __restrict__ dy,
+                            const float* __restrict__ rho, float* __restrict__ gv,
+                            float* __restrict__ gq, int rows, int D) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= D) return;
+    float a = 0.f, b = 0.f;
+    for (int r = 0; r < rows; ++r) {
+        const float z = Z[(size_t)r * D + c], g = dy[r];
+        const float p = rho ? rho[(size_t)r * D + c] : 0.f;
+        a += g * z + p;
+        b += g * z * z + 2.f * z * p;
+    }
+    gv[c] = a;
+    gq[c] = b;
+}
+
+// minibatch rows of the exact-gradient targets (padding rows zero)
+__global__ void k_gather_grad(const float* __restrict__ Gall, const int32_t* __restrict__ idx,
+                              float* __restrict__ G, int n, int rows, int D) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long long)rows * D) return;
+    const int r = (int)(i / D), c = (int)(i % D);
+    G[i] = r < n ? Gall[(size_t)idx[r] * D + c] : 0.f;
+}
+
 int pad_to(int x, int m) { return (x + m - 1) / m * m; }
 unsigned nblocks(long long n, int t = 256) { return (unsigned)((n + t - 1) / t); }
 
@@ -298,14 +402,17 @@ struct mcp_mlp {
     // activations (rows x ...)
     float *Z = nullptr, *H1 = nullptr, *H2 = nullptr, *y = nullptr, *t = nullptr, *dy = nullptr,
           *lrow = nullptr, *dA2 = nullptr, *dH1 = nullptr, *dZ = nullptr;
+    // gradient-matching training (rows x ...)
+    float *Gb = nullptr, *rho = nullptr, *D2 = nullptr, *T1 = nullptr, *Dl1 = nullptr,
+          *K1 = nullptr, *K2 = nullptr, *Gam1 = nullptr;
     double *dX = nullptr, *dOut = nullptr;  // rows x d, rows x (d+1)
     float* loss = nullptr;                  // 64 floats
     // feature and target scaling
     float *xmean = nullptr, *xsd = nullptr;
     std::vector<float> h_xmean, h_xsd;
     float ymean = 0.f, ysd = 1.f;
-    // resident training set
-    float *Zall = nullptr, *tall = nullptr;
+    // resident training set (Gall: standardised exact gradients, optional)
+    float *Zall = nullptr, *tall = nullptr, *Gall = nullptr;
     int32_t* idx = nullptr;
     int64_t ntrain = 0;
     // surrogate leapfrog
@@ -325,8 +432,9 @@ struct mcp_mlp {
     ~mcp_mlp() {
         if (stream) (void)hipStreamSynchronize(stream);
         drop_graph();
-        void* ptrs[] = {p,   gp,  am, av, Z,    H1,   H2,    y,   t,    dy,   lrow, dA2, dH1,
-                        dZ,  dX,  dOut, loss, xmean, xsd, Zall, tall, idx, dstate, dlp, ditems};
+        void* ptrs[] = {p,   gp,  am,  av,    Z,   H1,   H2,   y,   t,      dy,  lrow,   dA2,
+                        dH1, dZ,  dX,  dOut,  loss, xmean, xsd, Zall, tall, idx, dstate, dlp,
+                        ditems, Gb, rho, D2, T1, Dl1, K1, K2, Gam1, Gall};
         for (void* ptr : ptrs)
             if (ptr) (void)hipFree(ptr);
         if (hstate) (void)hipHostFree(hstate);
@@ -380,7 +488,11 @@ int ensure_rows(mcp_mlp* m, int need) {
                 {(void**)&m->t, R * 4},        {(void**)&m->dy, R * 4},
                 {(void**)&m->lrow, R * 4},     {(void**)&m->dA2, R * H * 4},
                 {(void**)&m->dH1, R * H * 4},  {(void**)&m->dZ, R * D * 4},
-                {(void**)&m->dX, R * d * 8},   {(void**)&m->dOut, R * (d + 1) * 8}};
+                {(void**)&m->dX, R * d * 8},   {(void**)&m->dOut, R * (d + 1) * 8},
+                {(void**)&m->Gb, R * D * 4},   {(void**)&m->rho, R * D * 4},
+                {(void**)&m->D2, R * H * 4},   {(void**)&m->T1, R * H * 4},
+                {(void**)&m->Dl1, R * H * 4},  {(void**)&m->K1, R * H * 4},
+                {(void**)&m->K2, R * H * 4},   {(void**)&m->Gam1, R * H * 4}};
     for (Buf& b : bufs) {
         if (*b.p) (void)hipFree(*b.p);
         *b.p = nullptr;
@@ -399,15 +511,15 @@ void gemm(mcp_mlp* m, int M, int N, int K, const float* A, int lda, const float*
                        m->stream, A, lda, B, ldb, C, ldc, K, aux);
 }
 
-// h1, h2, y over `rows` rows of Z; with seed_input also dA2 for dy = 1
-void forward(mcp_mlp* m, int rows, bool seed_input) {
+// h1, h2, y over `rows` rows of Z; with `seed` also the input-gradient seed
+// delta2 = w3 (1 - h2^2) into seed
+void forward(mcp_mlp* m, int rows, float* seed) {
     const int H = m->H, D = m->D;
     const float* P = m->p;
     gemm<false, true, EPI_BIAS_TANH>(m, rows, H, D, m->Z, D, P + m->oW1, D, m->H1, H, P + m->ob1);
     gemm<false, true, EPI_BIAS_TANH>(m, rows, H, H, m->H1, H, P + m->oW2, H, m->H2, H, P + m->ob2);
     hipLaunchKernelGGL(k_head, dim3((rows + 3) / 4), dim3(256), 0, m->stream, m->H2, H, m->Z, D,
-                       P + m->ow3, P + m->ob3, P + m->ov, P + m->oq, m->y,
-                       seed_input ? m->dA2 : nullptr, rows);
+                       P + m->ow3, P + m->ob3, P + m->ov, P + m->oq, m->y, seed, rows);
 }
 
 // input gradient dZ = ((dA2 W2) * (1 - h1^2)) W1 (without the skip terms)
@@ -417,25 +529,57 @@ void backward_input(mcp_mlp* m, int rows) {
     gemm<false, false, EPI_STORE>(m, rows, D, H, m->dH1, H, m->p + m->oW1, D, m->dZ, D, nullptr);
 }
 
-// parameter gradients of the batch loss into gp (every entry written)
-void backward_params(mcp_mlp* m, int rows, int n, float* loss_acc) {
+// parameter gradients of the batch loss into gp (every entry written); with
+// lam > 0 the gradient-matching term against the targets in Gb is included
+// (needs delta2 in D2 from forward()).
+void backward_params(mcp_mlp* m, int rows, int n, float* loss_acc, float lam) {
     const int H = m->H, D = m->D;
+    const float* W1 = m->p + m->oW1;
+    const float* W2 = m->p + m->oW2;
     float* G = m->gp;
     hipStream_t st = m->stream;
+    const long long tH = (long long)rows * H;
+    const unsigned bH = nblocks(tH);
     hipLaunchKernelGGL(k_head_bwd, dim3((rows + 3) / 4), dim3(256), 0, st, m->y, m->t, m->H2, H,
                        m->p + m->ow3, m->dy, m->dA2, m->lrow, n, rows);
+    if (lam > 0.f) {
+        // input gradient g = W1^T delta1 + v + 2 q z
+        gemm<false, false, EPI_STORE>(m, rows, H, H, m->D2, H, W2, H, m->T1, H, nullptr);
+        hipLaunchKernelGGL(k_sob_delta1, dim3(bH), dim3(256), 0, st, m->T1, m->H1, m->Dl1, tH);
+        gemm<false, false, EPI_STORE>(m, rows, D, H, m->Dl1, H, W1, D, m->dZ, D, nullptr);
+        hipLaunchKernelGGL(k_sob_rho, dim3((rows + 3) / 4), dim3(256), 0, st, m->dZ, m->Z, m->Gb,
+                           m->p + m->ov, m->p + m->oq, m->rho, m->lrow, D, m->d, n, rows,
+                           lam / ((float)n * (float)m->d));
+        // second backward pass through the input-gradient graph
+        gemm<false, true, EPI_STORE>(m, rows, H, D, m->rho, D, W1, D, m->K1, H, nullptr);
+        hipLaunchKernelGGL(k_sob_h1, dim3(bH), dim3(256), 0, st, m->K1, m->T1, m->H1, m->Gam1, tH);
+        gemm<false, true, EPI_STORE>(m, rows, H, H, m->K1, H, W2, H, m->K2, H, nullptr);
+        hipLaunchKernelGGL(k_sob_h2, dim3(bH), dim3(256), 0, st, m->K2, m->H2, m->D2, m->p + m->ow3,
+                           m->dy, m->dA2, H, tH);
+        hipLaunchKernelGGL(k_colsum, dim3(nblocks(H)), dim3(256), 0, st, m->K2, H, nullptr,
+                           G + m->ow3, nullptr, rows, H);
+    } else {
+        hipLaunchKernelGGL(k_colsum, dim3(nblocks(H)), dim3(256), 0, st, m->H2, H, m->dy,
+                           G + m->ow3, nullptr, rows, H);
+    }
     hipLaunchKernelGGL(k_sum2, dim3(1), dim3(64), 0, st, m->dy, m->lrow, G + m->ob3, loss_acc, rows);
-    hipLaunchKernelGGL(k_colsum, dim3(nblocks(H)), dim3(256), 0, st, m->H2, H, m->dy, G + m->ow3,
-                       nullptr, rows, H);
-    hipLaunchKernelGGL(k_colsum, dim3(nblocks(D)), dim3(256), 0, st, m->Z, D, m->dy, G + m->ov,
-                       G + m->oq, rows, D);
+    hipLaunchKernelGGL(k_colsum_vq, dim3(nblocks(D)), dim3(256), 0, st, m->Z, m->dy,
+                       lam > 0.f ? m->rho : nullptr, G + m->ov, G + m->oq, rows, D);
     hipLaunchKernelGGL(k_colsum, dim3(nblocks(H)), dim3(256), 0, st, m->dA2, H, nullptr, G + m->ob2,
                        nullptr, rows, H);
     gemm<true, false, EPI_STORE>(m, H, H, rows, m->dA2, H, m->H1, H, G + m->oW2, H, nullptr);
-    gemm<false, false, EPI_MUL_DTANH>(m, rows, H, H, m->dA2, H, m->p + m->oW2, H, m->dH1, H, m->H1);
+    if (lam > 0.f) {
+        gemm<true, false, EPI_ACCUM>(m, H, H, rows, m->D2, H, m->K1, H, G + m->oW2, H, nullptr);
+        gemm<false, false, EPI_STORE>(m, rows, H, H, m->dA2, H, W2, H, m->dH1, H, nullptr);
+        hipLaunchKernelGGL(k_sob_a1, dim3(bH), dim3(256), 0, st, m->dH1, m->Gam1, m->H1, tH);
+    } else {
+        gemm<false, false, EPI_MUL_DTANH>(m, rows, H, H, m->dA2, H, W2, H, m->dH1, H, m->H1);
+    }
     hipLaunchKernelGGL(k_colsum, dim3(nblocks(H)), dim3(256), 0, st, m->dH1, H, nullptr, G + m->ob1,
                        nullptr, rows, H);
     gemm<true, false, EPI_STORE>(m, H, D, rows, m->dH1, H, m->Z, D, G + m->oW1, D, nullptr);
+    if (lam > 0.f)
+        gemm<true, false, EPI_ACCUM>(m, H, D, rows, m->Dl1, H, m->rho, D, G + m->oW1, D, nullptr);
 }
 
 // logL~ and its gradient for the n rows of m->dX (already on the device)
@@ -443,7 +587,7 @@ void infer(mcp_mlp* m, int n) {
     const int rows = pad_to(n, kTile);
     hipLaunchKernelGGL(k_features, dim3(nblocks((long long)rows * m->D)), dim3(256), 0, m->stream,
                        m->dX, m->d, m->xmean, m->xsd, m->Z, rows, m->D, n);
-    forward(m, rows, true);
+    forward(m, rows, m->dA2);
     backward_input(m, rows);
     hipLaunchKernelGGL(k_finalize, dim3(nblocks((long long)n * (m->d + 1))), dim3(256), 0, m->stream,
                        m->y, m->dZ, m->Z, m->D, m->p + m->ov, m->p + m->oq, m->dX, m->d, m->xsd,
@@ -466,6 +610,20 @@ int features_host(mcp_mlp* m, const double* X, int64_t n, std::vector<float>& ou
         for (int c = 0; c < m->d; ++c)
             out[(size_t)r * m->D + c] =
                 (float)((std::log(X[r * m->d + c]) - (double)m->h_xmean[c]) / (double)m->h_xsd[c]);
+    return MCP_OK;
+}
+
+// standardised gradient targets dy/dz = dlogL/dx * x * x_sd / y_sd
+int grad_targets_host(mcp_mlp* m, const double* X, const double* grad, int64_t n,
+                      std::vector<float>& out) {
+    out.assign((size_t)n * m->D, 0.f);
+    for (int64_t r = 0; r < n; ++r)
+        for (int c = 0; c < m->d; ++c) {
+            const double g = grad[r * m->d + c];
+            if (!std::isfinite(g)) return set_last_error(MCP_ERR_INVALID, "non-finite gradient");
+            out[(size_t)r * m->D + c] =
+                (float)(g * X[r * m->d + c] * (double)m->h_xsd[c] / (double)m->ysd);
+        }
     return MCP_OK;
 }
 
@@ -609,23 +767,27 @@ int mcp_mlp_set_scaling(mcp_mlp* m, const float* x_mean, const float* x_sd, floa
     return MCP_OK;
 }
 
-int mcp_mlp_set_data(mcp_mlp* m, const double* X, const double* logl, int64_t n) {
+int mcp_mlp_set_data(mcp_mlp* m, const double* X, const double* logl, const double* grad,
+                     int64_t n) {
     if (!m || !X || !logl) return set_last_error(MCP_ERR_INVALID, "null argument");
     if (n < 1 || n > INT32_MAX) return set_last_error(MCP_ERR_INVALID, "bad number of rows");
     DeviceGuard guard(m->device);
-    std::vector<float> Zh;
+    std::vector<float> Zh, Gh;
     int st = features_host(m, X, n, Zh);
     if (st) return st;
+    if (grad && (st = grad_targets_host(m, X, grad, n, Gh))) return st;
     std::vector<float> th(n);
     for (int64_t i = 0; i < n; ++i) {
         if (!std::isfinite(logl[i])) return set_last_error(MCP_ERR_INVALID, "non-finite target");
         th[i] = (float)((logl[i] - (double)m->ymean) / (double)m->ysd);
     }
     MLP_TRY(hipStreamSynchronize(m->stream));
-    if (m->Zall) (void)hipFree(m->Zall);
-    if (m->tall) (void)hipFree(m->tall);
+    float** fb[] = {&m->Zall, &m->tall, &m->Gall};
+    for (float** q : fb) {
+        if (*q) (void)hipFree(*q);
+        *q = nullptr;
+    }
     if (m->idx) (void)hipFree(m->idx);
-    m->Zall = m->tall = nullptr;
     m->idx = nullptr;
     m->ntrain = 0;
     MLP_TRY(hipMalloc(&m->Zall, Zh.size() * sizeof(float)));
@@ -633,16 +795,24 @@ int mcp_mlp_set_data(mcp_mlp* m, const double* X, const double* logl, int64_t n)
     MLP_TRY(hipMalloc(&m->idx, (size_t)n * sizeof(int32_t)));
     MLP_TRY(h2d(m, m->Zall, Zh.data(), Zh.size() * sizeof(float)));
     MLP_TRY(h2d(m, m->tall, th.data(), th.size() * sizeof(float)));
+    if (grad) {
+        MLP_TRY(hipMalloc(&m->Gall, Gh.size() * sizeof(float)));
+        MLP_TRY(h2d(m, m->Gall, Gh.data(), Gh.size() * sizeof(float)));
+    }
     m->ntrain = n;
     return MCP_OK;
 }
 
 int mcp_mlp_train(mcp_mlp* m, int32_t batch, int32_t epochs, float lr, float lr_final,
-                  uint64_t seed, float* epoch_loss) {
+                  float grad_weight, uint64_t seed, float* epoch_loss) {
     if (!m) return set_last_error(MCP_ERR_INVALID, "null argument");
     if (m->ntrain < 1) return set_last_error(MCP_ERR_STATE, "no training data (mcp_mlp_set_data)");
-    if (batch < 1 || batch > (1 << 16) || epochs < 0 || !(lr > 0.f) || !(lr_final > 0.f))
+    if (batch < 1 || batch > (1 << 16) || epochs < 0 || !(lr > 0.f) || !(lr_final > 0.f) ||
+        !(grad_weight >= 0.f) || !std::isfinite(grad_weight))
         return set_last_error(MCP_ERR_INVALID, "bad training arguments");
+    if (grad_weight > 0.f && !m->Gall)
+        return set_last_error(MCP_ERR_STATE, "gradient weight > 0 needs gradient targets");
+    const float lam = grad_weight;
     DeviceGuard guard(m->device);
     int st = ensure_rows(m, batch);
     if (st) return st;
@@ -664,8 +834,11 @@ int mcp_mlp_train(mcp_mlp* m, int32_t batch, int32_t epochs, float lr, float lr_
             const int rows = pad_to(nb, kTile);
             hipLaunchKernelGGL(k_gather_rows, dim3(nblocks((long long)rows * m->D)), dim3(256), 0,
                                m->stream, m->Zall, m->idx + b0, m->tall, m->Z, m->t, nb, rows, m->D);
-            forward(m, rows, false);
-            backward_params(m, rows, nb, m->loss);
+            if (lam > 0.f)
+                hipLaunchKernelGGL(k_gather_grad, dim3(nblocks((long long)rows * m->D)), dim3(256),
+                                   0, m->stream, m->Gall, m->idx + b0, m->Gb, nb, rows, m->D);
+            forward(m, rows, lam > 0.f ? m->D2 : nullptr);
+            backward_params(m, rows, nb, m->loss, lam);
             ++m->adam_t;
             const double c1 = 1.0 - std::pow((double)b1, (double)m->adam_t);
             const double c2 = 1.0 - std::pow((double)b2, (double)m->adam_t);
@@ -685,10 +858,13 @@ int mcp_mlp_train(mcp_mlp* m, int32_t batch, int32_t epochs, float lr, float lr_
     return MCP_OK;
 }
 
-int mcp_mlp_param_grad(mcp_mlp* m, const double* X, const double* logl, int64_t n, float* grad,
-                       float* loss) {
+int mcp_mlp_param_grad(mcp_mlp* m, const double* X, const double* logl, const double* dlogl,
+                       int64_t n, float grad_weight, float* grad, float* loss) {
     if (!m || !X || !logl || !grad) return set_last_error(MCP_ERR_INVALID, "null argument");
     if (n < 1 || n > (1 << 16)) return set_last_error(MCP_ERR_INVALID, "bad number of rows");
+    if (!(grad_weight >= 0.f) || !std::isfinite(grad_weight) || (grad_weight > 0.f && !dlogl))
+        return set_last_error(MCP_ERR_INVALID, "gradient weight > 0 needs gradient targets");
+    const float lam = grad_weight;
     DeviceGuard guard(m->device);
     int st = ensure_rows(m, (int)n);
     if (st) return st;
@@ -700,9 +876,15 @@ int mcp_mlp_param_grad(mcp_mlp* m, const double* X, const double* logl, int64_t 
     for (int64_t i = 0; i < n; ++i) th[i] = (float)((logl[i] - (double)m->ymean) / (double)m->ysd);
     MLP_TRY(h2d(m, m->Z, Zh.data(), Zh.size() * sizeof(float)));
     MLP_TRY(h2d(m, m->t, th.data(), th.size() * sizeof(float)));
+    if (lam > 0.f) {
+        std::vector<float> Gh;
+        if ((st = grad_targets_host(m, X, dlogl, n, Gh))) return st;
+        Gh.resize((size_t)rows * m->D, 0.f);
+        MLP_TRY(h2d(m, m->Gb, Gh.data(), Gh.size() * sizeof(float)));
+    }
     MLP_TRY(hipMemsetAsync(m->loss, 0, sizeof(float), m->stream));
-    forward(m, rows, false);
-    backward_params(m, rows, (int)n, m->loss);
+    forward(m, rows, lam > 0.f ? m->D2 : nullptr);
+    backward_params(m, rows, (int)n, m->loss, lam);
     MLP_TRY(hipGetLastError());
     MLP_TRY(d2h(m, grad, m->gp, m->nparam * sizeof(float)));
     if (loss) MLP_TRY(d2h(m, loss, m->loss, sizeof(float)));

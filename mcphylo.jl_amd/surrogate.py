@@ -111,30 +111,39 @@ class Surrogate:
         self.set_scaling(L.mean(axis=0), np.maximum(L.std(axis=0), sd_floor), y.mean(),
                          max(float(y.std()), sd_floor))
 
-    def set_data(self, X, logl) -> None:
+    def set_data(self, X, logl, grad=None) -> None:
+        """Resident training set: branch lengths X [n][d], logL [n] and
+        optionally the exact gradients d logL / dx [n][d]."""
         X = _rows(X, self.d)
         y = np.ascontiguousarray(logl, dtype=np.float64).reshape(-1)
         if y.shape[0] != X.shape[0]:
             raise ValueError("X and logl have different numbers of rows")
-        check(_lib.load_library().mcp_mlp_set_data(self._h, dptr(X), dptr(y), X.shape[0]))
+        G = None if grad is None else _rows(grad, self.d)
+        if G is not None and G.shape != X.shape:
+            raise ValueError("grad must have the shape of X")
+        check(_lib.load_library().mcp_mlp_set_data(self._h, dptr(X), dptr(y), dptr(G),
+                                                   X.shape[0]))
 
     def train(self, epochs: int, batch: int = 512, lr: float = 1e-3, lr_final: float = 1e-5,
-              seed: int = 0) -> np.ndarray:
-        """Adam epochs over the resident data; returns the mean batch loss per
-        epoch (standardised units)."""
+              seed: int = 0, grad_weight: float = 0.0) -> np.ndarray:
+        """Adam epochs over the resident data (grad_weight > 0 adds gradient
+        matching); returns the mean batch loss per epoch (standardised units)."""
         losses = np.zeros(max(1, int(epochs)), dtype=np.float32)
         check(_lib.load_library().mcp_mlp_train(self._h, int(batch), int(epochs), float(lr),
-                                                float(lr_final), int(seed), fptr(losses)))
+                                                float(lr_final), float(grad_weight), int(seed),
+                                                fptr(losses)))
         return losses[:int(epochs)].astype(np.float64)
 
-    def param_grad(self, X, logl):
-        """(flat gradient of (1/2n) sum (y - t)^2, loss) over the rows of X."""
+    def param_grad(self, X, logl, grad=None, grad_weight: float = 0.0):
+        """(flat gradient of the training loss, loss) over the rows of X."""
         X = _rows(X, self.d)
         y = np.ascontiguousarray(logl, dtype=np.float64).reshape(-1)
+        G = None if grad is None else _rows(grad, self.d)
         g = np.zeros(self.n_params, dtype=np.float32)
         loss = np.zeros(1, dtype=np.float32)
-        check(_lib.load_library().mcp_mlp_param_grad(self._h, dptr(X), dptr(y), X.shape[0],
-                                                     fptr(g), fptr(loss)))
+        check(_lib.load_library().mcp_mlp_param_grad(self._h, dptr(X), dptr(y), dptr(G),
+                                                     X.shape[0], float(grad_weight), fptr(g),
+                                                     fptr(loss)))
         return g, float(loss[0])
 
     # -- inference ---------------------------------------------------------------

@@ -8,7 +8,9 @@ network and its gradients in numpy so the device kernels can be checked:
     z  = (log x - x_mean) / x_sd                       (features)
     h1 = tanh(W1 z + b1);  h2 = tanh(W2 h1 + b2)
     y  = w3.h2 + b3[0] + v.z + q.(z*z);   logL~ = y y_sd + y_mean
-    loss = (1/2n) sum_r (y_r - t_r)^2,   t = (logL - y_mean) / y_sd
+    loss = (1/2n) sum_r (y_r - t_r)^2 + (lam / 2nd) sum_{r, c<d} (g_rc - G_rc)^2
+    t = (logL - y_mean) / y_sd,  g = dy/dz (input gradient),
+    G = dlogL/dx * x * x_sd / y_sd (standardised exact gradient)
 
 Two modes:
   exact  -- every product in float64 (the mathematical network);
@@ -18,7 +20,8 @@ Two modes:
             points, so only FP32 accumulation order separates it from the
             device.
 Pinned by central finite differences of the exact mode (tests/test_surrogate
-_oracle.py): parameter gradients of the loss and input gradients of logL~.
+_oracle.py): parameter gradients of the loss (with and without the
+gradient-matching term) and input gradients of logL~.
 """
 
 from __future__ import annotations
@@ -116,6 +119,51 @@ class MLPOracle:
         loss = 0.5 * float(e @ e) / n
         return loss, np.concatenate([np.asarray(g[k], np.float64).reshape(-1) for k in BLOCKS])
 
+    def loss_grad_sobolev(self, Z, t, GZ, lam, d):
+        """(loss, flat gradient) with the gradient-matching term: a second
+        backward pass through the input-gradient graph
+        (delta2 = w3 s2, T1 = delta2 W2, delta1 = T1 s1, g = delta1 W1 + v + 2qz)."""
+        p = self.p
+        P = self.p32 if self.mode == "bf16" else p
+        st, mm = self._store, self._mm
+        n = Z.shape[0]
+        h1, h2, y = self.forward(Z)
+        h1d, h2d = np.asarray(h1, np.float64), np.asarray(h2, np.float64)
+        s1, s2 = 1.0 - h1d * h1d, 1.0 - h2d * h2d
+        Z64 = Z.astype(np.float64)
+        e = np.asarray(y, np.float64) - t
+        dyd = np.asarray(st(e / n), np.float64)
+        D2 = st(p["w3"][None, :] * s2)
+        T1 = st(mm(D2, P["W2"]))
+        Dl1 = st(np.asarray(T1, np.float64) * s1)
+        dZ = np.asarray(st(mm(Dl1, P["W1"])), np.float64)
+        gin = dZ + p["v"][None, :] + 2.0 * p["q"][None, :] * Z64
+        mask = np.zeros(Z.shape[1])
+        mask[:d] = 1.0
+        diff = (gin - np.asarray(GZ, np.float64)) * mask[None, :]
+        lam_n = lam / (n * d)
+        rho = st(lam_n * diff)
+        rhod = np.asarray(rho, np.float64)
+        K1 = np.asarray(st(mm(rho, P["W1"].T)), np.float64)
+        mu1 = st(K1 * s1)
+        gam1 = -2.0 * h1d * K1 * np.asarray(T1, np.float64)
+        K2 = np.asarray(st(mm(mu1, P["W2"].T)), np.float64)
+        D2d = np.asarray(D2, np.float64)
+        A2 = st(dyd[:, None] * D2d - 2.0 * h2d * K2 * p["w3"][None, :] * s2)
+        g = {
+            "w3": (dyd[:, None] * h2d + K2 * s2).sum(axis=0),
+            "v": (dyd[:, None] * Z64 + rhod).sum(axis=0),
+            "q": (dyd[:, None] * Z64 * Z64 + 2.0 * Z64 * rhod).sum(axis=0),
+            "b3": np.concatenate([[dyd.sum()], np.zeros(63)]),
+            "b2": np.asarray(A2, np.float64).sum(axis=0),
+            "W2": mm(np.asarray(A2).T, h1) + mm(np.asarray(D2).T, mu1),
+        }
+        A1 = st((mm(A2, P["W2"]) + gam1) * s1)
+        g["b1"] = np.asarray(A1, np.float64).sum(axis=0)
+        g["W1"] = mm(np.asarray(A1).T, Z) + mm(np.asarray(Dl1).T, rho)
+        loss = 0.5 * float(e @ e) / n + 0.5 * lam_n * float(np.sum(diff * diff))
+        return loss, np.concatenate([np.asarray(g[k], np.float64).reshape(-1) for k in BLOCKS])
+
     def input_grad(self, Z):
         """d y / d z per row (float64)."""
         p = self.p
@@ -143,12 +191,23 @@ def predict(flat, hidden, D, X, x_mean, x_sd, y_mean, y_sd, mode="exact"):
     return ll, ys * gz / (sd[None, :] * X)
 
 
-def param_grad(flat, hidden, D, X, logl, x_mean, x_sd, y_mean, y_sd, mode="exact"):
-    """(loss, flat gradient) of the training loss over the rows of (X, logl)."""
+def param_grad(flat, hidden, D, X, logl, x_mean, x_sd, y_mean, y_sd, mode="exact",
+               dlogl=None, grad_weight=0.0):
+    """(loss, flat gradient) of the training loss over the rows of (X, logl)
+    and, with grad_weight > 0, the exact gradients dlogl [n][d]."""
     X = np.atleast_2d(np.asarray(X, dtype=np.float64))
     Z = features(X, x_mean, x_sd, D, np.float32 if mode == "bf16" else np.float64)
     ym, ys = float(np.float32(y_mean)), float(np.float32(y_sd))
     t = (np.asarray(logl, np.float64) - ym) / ys
     if mode == "bf16":
         t = t.astype(np.float32).astype(np.float64)
-    return MLPOracle(flat, hidden, D, mode).loss_grad(Z, t)
+    o = MLPOracle(flat, hidden, D, mode)
+    if grad_weight <= 0.0:
+        return o.loss_grad(Z, t)
+    n, d = X.shape
+    GZ = np.zeros((n, D))
+    GZ[:, :d] = (np.asarray(dlogl, np.float64) * X
+                 * np.asarray(x_sd, np.float32).astype(np.float64) / ys)
+    if mode == "bf16":
+        GZ = GZ.astype(np.float32).astype(np.float64)
+    return o.loss_grad_sobolev(Z, t, GZ, float(np.float32(grad_weight)), d)

@@ -248,3 +248,22 @@ def test_delayed_acceptance_matches_plain_nuts_and_saves_exact_evaluations():
     per_da = st["exact_evals"] / ess_da
     per_pl = exact_plain / ess_pl
     assert per_pl > 3.0 * per_da, (per_pl, per_da)
+
+
+@pytest.mark.parametrize("d,H,n,lam", [(17, 256, 100, 2.0), (398, 256, 200, 50.0)])
+def test_gradient_matching_param_grad_matches_oracle(d, H, n, lam):
+    s, flat, X, rng = _random_surrogate(d, H, 3 * d + n)
+    X = X[:n]
+    logl = -1234.5 + 6.5 * rng.standard_normal(n)
+    dlogl = rng.normal(0, 30, (n, d))
+    g, loss = s.param_grad(X, logl, grad=dlogl, grad_weight=lam)
+    args = (flat, H, s.D, X, logl, s.x_mean, s.x_sd, s.y_mean, s.y_sd)
+    l_bf, g_bf = mo.param_grad(*args, mode="bf16", dlogl=dlogl, grad_weight=lam)
+    l_ex, g_ex = mo.param_grad(*args, dlogl=dlogl, grad_weight=lam)
+    _blocks_close(s, g, g_bf, 2e-3)
+    assert abs(loss - l_bf) <= 1e-4 * l_bf
+    _blocks_close(s, g, g_ex, 5e-2)
+    # weight 0 with gradients given equals the value-only loss
+    g0, l0 = s.param_grad(X, logl, grad=dlogl, grad_weight=0.0)
+    g1, l1 = s.param_grad(X, logl)
+    assert np.array_equal(g0, g1) and l0 == l1

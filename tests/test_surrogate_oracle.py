@@ -114,3 +114,35 @@ def test_bf16_mode_is_close_to_exact():
     l2, g2 = mo.param_grad(flat, H, D, X, logl, xm, xs, logl.mean(), logl.std(), mode="bf16")
     assert abs(l1 - l2) < 2e-2 * l1
     assert np.linalg.norm(g1 - g2) < 2e-2 * np.linalg.norm(g1)
+
+
+def test_gradient_matching_loss_matches_finite_differences():
+    rng = np.random.default_rng(4)
+    d, H, n = 9, 64, 11
+    flat, D = _random_net(rng, d, H)
+    X = np.exp(rng.normal(-2.5, 0.6, (n, d)))
+    xm, xs = np.log(X).mean(0), np.log(X).std(0)
+    logl = rng.normal(-100, 3, n)
+    dlogl = rng.normal(0, 20, (n, d))
+    ym, ys = logl.mean(), logl.std()
+    kw = dict(dlogl=dlogl, grad_weight=3.0)
+    loss, g = mo.param_grad(flat, H, D, X, logl, xm, xs, ym, ys, **kw)
+    l0, _ = mo.param_grad(flat, H, D, X, logl, xm, xs, ym, ys)
+    assert loss > l0                       # the matching term adds a positive amount
+    offs = np.cumsum([0, H * D, H, H * H, H, H, 64, D, D])
+    valid = [[r * D + c for r in range(H) for c in range(d)], range(H), range(H * H), range(H),
+             range(H), [0], range(d), range(d)]
+    picks = [offs[b] + int(j) for b, idxs in enumerate(valid)
+             for j in rng.choice(list(idxs), size=min(5, len(idxs)), replace=False)]
+    h = 1e-6
+    for i in picks:
+        fp, fm = flat.copy(), flat.copy()
+        fp[i] += h
+        fm[i] -= h
+        lp, _ = mo.param_grad(fp, H, D, X, logl, xm, xs, ym, ys, **kw)
+        lm, _ = mo.param_grad(fm, H, D, X, logl, xm, xs, ym, ys, **kw)
+        fd = (lp - lm) / (2 * h)
+        assert abs(fd - g[i]) <= 1e-6 * max(1.0, abs(g[i])), (i, fd, g[i])
+    # with weight 0 the two paths agree
+    l1, g1 = mo.param_grad(flat, H, D, X, logl, xm, xs, ym, ys, dlogl=dlogl, grad_weight=0.0)
+    assert l1 == l0
