@@ -171,6 +171,122 @@ def bench_nuts(n_adapt, n_iter, cpu: bool):
     return out
 
 
+def bench_da(n_warm, n_iter, n_chains=8, prior=0.1, depth=8, grad_weight=1e4, epochs=30):
+    """Delayed-acceptance NUTS on the 200-taxon GTR workload.  A shared warm-up
+    (plain NUTS, exact device leapfrog, molifier off) records every leapfrog's
+    (x, logL, dlogL/dx) after its first third; plain NUTS and surrogate DA-NUTS
+    then continue from the warm-up's states and step sizes for n_iter
+    iterations each.  ESS by batch means (batch 25) over the pooled chains."""
+    S = mcp.sampler
+    w = mcp.synthetic.make_workload("gtr200")
+    ta = w.ta
+    d = ta.parent.shape[0] - 1
+    model = mcp.Model(w.U, w.D, w.Uinv, w.mu, w.rates, w.pi)
+    b = mcp.Batch([{"x_leaves": w.x[:, :, ta.leaf_nums - 1], "leaf_nums": ta.leaf_nums,
+                    "weights": w.weights}],
+                  [{"dataset": 0, "parent": ta.parent, "n_rates": len(w.rates)}] * n_chains)
+    for i in range(n_chains):
+        b.set_model(i, model)
+        b.set_prior_exponential(i, prior)
+    b.use_graph(True)
+
+    class Recording(S.EngineLeapfrog):
+        """Exact leapfrog that keeps (x, logL, dlogL/dx) of its steps."""
+        on = False
+
+        def __call__(self, reqs):
+            out = super().__call__(reqs)
+            if self.on:
+                for q, st, ll in zip(reqs, out, self.last_ll):
+                    if q[3] != 0.0:   # molifier off: g = dlogL/dx - 1/prior
+                        self.pairs.append((st.x, float(ll), st.g + 1.0 / prior))
+            return out
+
+    lf = Recording(b, n_chains, [d] * n_chains)
+    lf.pairs = []
+    engine_leapfrog = b.leapfrog
+
+    def leapfrog_keep_ll(ed, x, r, g):
+        lp, ll = engine_leapfrog(ed, x, r, g)
+        lf.last_ll = ll
+        return lp, ll
+    b.leapfrog = leapfrog_keep_ll
+    blv = mcp.get_branchlength_vector(w.tree)
+    warm = [S.NUTSChain(blv, np.random.default_rng(100 + i), target=0.8, tree_depth=depth,
+                        molifier=False) for i in range(n_chains)]
+
+    def tr(i, k):
+        if i == 0 and k == n_warm // 3:
+            lf.on = True
+        if i == 0 and k % 25 == 0:
+            log(f"  warm-up iteration {k}")
+    t0 = time.perf_counter()
+    S.run_nuts(warm, lf, 0, n_adapt=n_warm, trace=tr)
+    warm_s = time.perf_counter() - t0
+    lf.on = False
+    X = np.array([p[0] for p in lf.pairs])
+    L = np.array([p[1] for p in lf.pairs])
+    G = np.array([p[2] for p in lf.pairs])
+    log(f"  warm-up {warm_s:.1f} s, {len(L)} training pairs")
+
+    def summary(draws, lps, wall):
+        tl = np.concatenate([c.sum(axis=1) for c in draws])
+        ess_tl, ess_lp = S.ess_bm(tl, 25), S.ess_bm(np.concatenate(lps), 25)
+        ess_b = min(S.ess_bm(np.concatenate([c[:, k] for c in draws]), 25) for k in range(d))
+        return {"wall_s": wall, "ess_tree_length": ess_tl, "ess_logpost": ess_lp,
+                "min_ess_branch": ess_b, "ess_per_s": min(ess_tl, ess_lp) / wall,
+                "mean_tree_length": float(tl.mean())}
+
+    plain = [S.NUTSChain(c.x, np.random.default_rng(200 + i), target=0.8, tree_depth=depth,
+                         epsilon=c.epsilon, molifier=False) for i, c in enumerate(warm)]
+    dp, lpp, wall_p, rounds_p = S.run_nuts(plain, lf, n_iter, n_adapt=0)
+    out = {"workload": f"{len(ta.leaf_nums)} taxa, 4-state GTR, {w.C} weighted patterns, "
+                       f"4 gamma cats, exponential({prior}) branch-length prior, {n_chains} chains "
+                       f"in lockstep, NUTS target 0.8, tree depth {depth}, molifier off",
+           "warmup_iterations": n_warm, "iterations": n_iter, "training_pairs": int(len(L)),
+           "plain": summary(dp, lpp, wall_p)}
+    out["plain"]["exact_leapfrogs"] = int(sum(c.leapfrogs for c in plain))
+    log(f"  plain: {wall_p:.1f} s, ESS/s {out['plain']['ess_per_s']:.2f}")
+    s = mcp.Surrogate(d, 256, 0, 1)
+    s.fit_scaling(X, L)
+    t1 = time.perf_counter()
+    s.set_data(X, L, grad=G)
+    losses = s.train(epochs, batch=256, lr=1e-3, lr_final=1e-5, seed=2, grad_weight=grad_weight)
+    train_s = time.perf_counter() - t1
+    Dp = np.concatenate(dp)
+    err = s.predict(Dp) - (np.concatenate(lpp) - S.exponential_logprior(Dp, prior))
+    s.use_graph(True)
+    da = [S.NUTSChain(c.x, np.random.default_rng(300 + i), target=0.8, tree_depth=depth,
+                      epsilon=c.epsilon, molifier=False) for i, c in enumerate(warm)]
+
+    def exact_logpost(Xs):
+        return b.logl(Xs.reshape(-1)) + S.exponential_logprior(Xs, prior)
+    dd, lpd, wall_d, st = S.run_da_nuts(da, S.SurrogateLeapfrog(s, n_chains, prior),
+                                        exact_logpost, n_iter, n_adapt=0)
+    out["da"] = summary(dd, lpd, wall_d)
+    out["da"].update({"second_stage_accept": st["accept_rate"],
+                      "exact_evaluations": int(st["exact_evals"]),
+                      "surrogate_leapfrog_rounds": int(st["rounds"])})
+    A = np.concatenate(dd)
+    z = np.array([abs(A[:, k].mean() - Dp[:, k].mean())
+                  / np.hypot(S.mcse_bm(A[:, k], 25), S.mcse_bm(Dp[:, k], 25)) for k in range(d)])
+    out["surrogate"] = {"network": f"{d} -> 256 -> 256 -> 1 tanh MLP + separable quadratic skip, "
+                                   "BF16 MFMA / FP32 accumulation",
+                        "training": f"{epochs} epochs Adam, batch 256, value + gradient matching "
+                                    f"(weight {grad_weight:g})",
+                        "train_s": train_s, "final_loss": float(losses[-1]),
+                        "error_sd_on_plain_draws": float(err.std()),
+                        "error_mean_on_plain_draws": float(err.mean())}
+    out["posterior_agreement"] = {"branches": d, "frac_abs_z_gt_3": float(np.mean(z > 3)),
+                                  "max_abs_z": float(z.max())}
+    out["da_over_plain_ess_per_s"] = out["da"]["ess_per_s"] / out["plain"]["ess_per_s"]
+    out["da_over_plain_including_training"] = (
+        min(out["da"]["ess_tree_length"], out["da"]["ess_logpost"]) / (wall_d + train_s)
+        / out["plain"]["ess_per_s"])
+    b.close()
+    return out
+
+
 def bench_dense(n_sites, steps, cpu: bool, cpu_seconds: float):
     """The dense 20-state workload (1000 taxa, n_sites sites as leaf state
     codes, 4 gamma categories) on the FP64-MFMA level kernels; the CPU oracle
@@ -283,6 +399,8 @@ def main():
                     help="run only the ragged-batch section (for profiling it alone)")
     ap.add_argument("--schedule", default="levels", choices=["levels", "fused"],
                     help="launch schedule of the timed evaluations")
+    ap.add_argument("--da-iters", type=int, default=100,
+                    help="iterations of the delayed-acceptance section (200-taxon GTR); 0 skips")
     ap.add_argument("--dense-sites", type=int, default=100_000,
                     help="sites of the dense 20-state section (1000 taxa); 0 skips it")
     ap.add_argument("--dense-steps", type=int, default=5)
@@ -417,6 +535,10 @@ def main():
         log("NUTS ESS/s section")
         result["nuts"] = bench_nuts(100, args.nuts_iters, not args.no_cpu)
         log(f"nuts: gpu {result['nuts']['gpu']['ess_per_s']:.1f} ESS/s")
+    if args.da_iters > 0:
+        log("delayed-acceptance section")
+        result["delayed_acceptance"] = bench_da(100, args.da_iters)
+        log(f"DA: {result['delayed_acceptance']['da_over_plain_ess_per_s']:.2f}x plain ESS/s")
     if args.dense_sites > 0:
         log("dense 20-state section")
         result["dense20"] = bench_dense(args.dense_sites, args.dense_steps, not args.no_cpu,

@@ -417,14 +417,15 @@ def exponential_logprior(X, scale: float) -> np.ndarray:
 
 def run_da_nuts(chains, leapfrog, exact_logpost, n_iter: int, n_adapt: int = 0,
                 min_accept: float = 0.0, window: int = 50, fallback=None, trace=None,
-                record=None):
+                record=None, stage1: int = 1):
     """Delayed-acceptance NUTS.  Stage one: a NUTS transition of every chain
     integrated on the surrogate posterior by `leapfrog` (a SurrogateLeapfrog);
     its draw x' is a proposal.  Stage two: one batched exact evaluation
     `exact_logpost(X) -> [n_chains]` and the Metropolis-Hastings correction
         log a = (lp(x') - lp~(x')) - (lp(x) - lp~(x)),
     which leaves the exact posterior invariant because the NUTS transition is
-    reversible with respect to the surrogate posterior.  lp~ of the current
+    reversible with respect to the surrogate posterior (so is `stage1` > 1
+    repetitions of it, a power of a self-adjoint kernel).  lp~ of the current
     state is the value the surrogate leapfrog returned for it (the first one
     from a zero-length step with the molifier off), so both stages compare
     values from the same kernels.
@@ -452,7 +453,8 @@ def run_da_nuts(chains, leapfrog, exact_logpost, n_iter: int, n_adapt: int = 0,
     while it < total:
         adapt = it < n_adapt
         x_old = [c.x.copy() for c in chains]
-        stats["rounds"] += _lockstep_iteration(chains, leapfrog, adapt)
+        for _ in range(max(1, int(stage1))):
+            stats["rounds"] += _lockstep_iteration(chains, leapfrog, adapt)
         X = np.stack([c.x for c in chains])
         lp_new = np.asarray(exact_logpost(X), dtype=np.float64)
         stats["exact_evals"] += n
