@@ -404,6 +404,8 @@ def main():
     ap.add_argument("--dense-sites", type=int, default=100_000,
                     help="sites of the dense 20-state section (1000 taxa); 0 skips it")
     ap.add_argument("--dense-steps", type=int, default=5)
+    ap.add_argument("--only-dense", action="store_true",
+                    help="run only the dense 20-state section (for profiling it alone)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -419,6 +421,11 @@ def main():
     if mcp._lib.device_count() < 1:
         log("no HIP device visible")
         sys.exit(1)
+    if args.only_dense:
+        res = bench_dense(args.dense_sites, args.dense_steps, not args.no_cpu,
+                          min(args.cpu_seconds, 8.0))
+        print(json.dumps({"dense20": res}), flush=True)
+        return
     if args.only_batched:
         if batch_data is None:
             log("--only-batched needs --batch-trees > 0")

@@ -31,7 +31,7 @@ hipError_t launch_level(bool, int, const LevelJob*, const int32_t*, const int32_
 hipError_t launch_fused(bool, int, const ItemSched*, int, int, int, const LevelJob*, const int32_t*,
                         const LevelJob*, const int32_t*, double*, hipStream_t);
 int cols_per_thread(int S);
-hipError_t launch_reduce(const RedOp*, int, const double*, double*, hipStream_t);
+hipError_t launch_reduce(const RedOp*, int, const double*, double*, double*, hipStream_t);
 bool supported_states(int S);
 hipError_t launch_lf_pre(const HmcItem*, int, double*, double*, const double*, const double*,
                          double*, int, hipStream_t);
@@ -208,7 +208,7 @@ struct mcp_engine {
     // device arenas
     DevBuf dF, dG0, dG1, dP, ddP, dPT, ddPT, dparts, dout, dblv, dmodel, dcodes, dweights, dtiptab,
         dleafof, dmodels, djobs_f, djobs_a, dmore, dtaskoff, dred, delemP, delemT, dsched, dlvl_f,
-        dlvl_a, dblkjob;
+        dlvl_a, dblkjob, dred2, dredtmp;
     PinnedBuf hblv, hout;
     // schedule
     std::vector<Launch> fwd, adj;
@@ -217,8 +217,11 @@ struct mcp_engine {
         int S, sched_begin, nitems, nblocks;
     };
     std::vector<FusedLaunch> fused;
-    std::vector<RedOp> red;  // ll ops first (n_items), then gradient ops
-    int n_ll_ops = 0;
+    // reductions: stage 1 over the partial sums (ll ops first, then gradient
+    // ops; long ones split into chunks into a temporary), stage 2 combining
+    // the chunks (ll ops first)
+    std::vector<RedOp> red, red2;
+    int n_ll_ops = 0, n_ll_ops2 = 0;
     long long nelemP = 0, nelemT = 0;
     size_t total_blv = 0, total_out = 0;
     long long total_patterns = 0;
@@ -790,26 +793,34 @@ static int rebuild_schedule(mcp_engine* e) {
     if ((st = upload(e, e->dblkjob, e->blkjob.data(), e->blkjob.size() * sizeof(int32_t))))
         return st;
 
-    // reductions: ll of every item first, then every branch gradient
+    // reductions: ll of every item first, then every branch gradient; sums
+    // longer than kRedChunk are cut into fixed chunks (stage 1 into a
+    // temporary, stage 2 over the chunk sums) so no block sums millions
     e->red.clear();
-    for (auto& it : e->items) {
-        RedOp op;
-        op.dst = it.out_off;
-        op.start = it.parts_off;
-        op.len = it.nint * it.R * it.ntiles;
-        op.pad_ = 0;
-        e->red.push_back(op);
-    }
-    e->n_ll_ops = (int)e->red.size();
-    for (auto& it : e->items)
-        for (int v = 0; v < it.N - 1; ++v) {
-            RedOp op;
-            op.dst = it.out_off + 1 + v;
-            op.start = it.parts_off + (it.nint + v) * it.R * it.ntiles;
-            op.len = it.R * it.ntiles;
-            op.pad_ = 0;
-            e->red.push_back(op);
+    e->red2.clear();
+    const int32_t kRedChunk = 1 << 15;
+    int32_t ntmp = 0;
+    auto add_sum = [&](int32_t dst, int32_t start, int32_t len) {
+        if (len <= kRedChunk) {
+            e->red.push_back(RedOp{dst, start, len, 0});
+            return;
         }
+        const int32_t first = ntmp;
+        for (int32_t a = 0; a < len; a += kRedChunk)
+            e->red.push_back(RedOp{ntmp++, start + a, std::min(kRedChunk, len - a), 1});
+        e->red2.push_back(RedOp{dst, first, ntmp - first, 0});
+    };
+    for (auto& it : e->items) add_sum(it.out_off, it.parts_off, it.nint * it.R * it.ntiles);
+    e->n_ll_ops = (int)e->red.size();
+    e->n_ll_ops2 = (int)e->red2.size();
+    for (auto& it : e->items)
+        for (int v = 0; v < it.N - 1; ++v)
+            add_sum(it.out_off + 1 + v, it.parts_off + (it.nint + v) * it.R * it.ntiles,
+                    it.R * it.ntiles);
+    if (!e->red2.empty()) {
+        if ((st = upload(e, e->dred2, e->red2.data(), e->red2.size() * sizeof(RedOp)))) return st;
+        if ((st = e->dredtmp.ensure((size_t)ntmp * sizeof(double)))) return st;
+    }
     if ((st = upload(e, e->dred, e->red.data(), e->red.size() * sizeof(RedOp)))) return st;
     return MCP_OK;
 }
@@ -1023,8 +1034,13 @@ static int enqueue_eval(mcp_engine* e, bool grad) {
                            e->dparts.as<double>(), st));
     }
     const int nops = grad ? (int)e->red.size() : e->n_ll_ops;
+    const int nops2 = grad ? (int)e->red2.size() : e->n_ll_ops2;
     TIMED(4, launch_reduce(e->dred.as<RedOp>(), nops, e->dparts.as<double>(),
-                           e->dout.as<double>(), st));
+                           e->dout.as<double>(), e->dredtmp.as<double>(), st));
+    if (nops2 > 0) {
+        TIMED(4, launch_reduce(e->dred2.as<RedOp>(), nops2, e->dredtmp.as<double>(),
+                               e->dout.as<double>(), e->dredtmp.as<double>(), st));
+    }
 #undef TIMED
     HIP_TRY(hipMemcpyAsync(e->hout.p, e->dout.p, e->total_out * sizeof(double),
                            hipMemcpyDeviceToHost, st));
@@ -1150,7 +1166,9 @@ static int enqueue_leapfrog(mcp_engine* e) {
                              e->dlvl_f.as<int32_t>(), e->djobs_a.as<LevelJob>(),
                              e->dlvl_a.as<int32_t>(), e->dparts.as<double>(), st));
     HIP_TRY(launch_reduce(e->dred.as<RedOp>(), (int)e->red.size(), e->dparts.as<double>(),
-                          e->dout.as<double>(), st));
+                          e->dout.as<double>(), e->dredtmp.as<double>(), st));
+    HIP_TRY(launch_reduce(e->dred2.as<RedOp>(), (int)e->red2.size(), e->dredtmp.as<double>(),
+                          e->dout.as<double>(), e->dredtmp.as<double>(), st));
     HIP_TRY(launch_lf_post(e->dhmc_items.as<HmcItem>(), n, dx, dr, dg, ded, e->dout.as<double>(),
                            e->dlp.as<double>(), st));
     HIP_TRY(hipMemcpyAsync(h, dx, 3 * T * sizeof(double), hipMemcpyDeviceToHost, st));

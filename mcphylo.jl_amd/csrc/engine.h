@@ -75,11 +75,14 @@ struct ItemSched {
 };
 
 // Reduction descriptor: out[dst] = sum(parts[start .. start+len)).
+// One fixed-order sum: out[dst] (or tmp[dst] when to_tmp) = sum of
+// src[start .. start+len).  Long sums are split into chunks summed into tmp
+// by a first launch and combined by a second, in a fixed order.
 struct RedOp {
     int32_t dst;
     int32_t start;
     int32_t len;
-    int32_t pad_;
+    int32_t to_tmp;
 };
 
 }  // namespace mcp

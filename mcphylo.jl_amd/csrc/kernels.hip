@@ -447,7 +447,8 @@ __global__ void __launch_bounds__(256) k_fused(const ItemSched* __restrict__ ite
 
 __global__ void __launch_bounds__(256) k_reduce(const RedOp* __restrict__ ops, int nops,
                                                 const double* __restrict__ parts,
-                                                double* __restrict__ out) {
+                                                double* __restrict__ out,
+                                                double* __restrict__ tmp) {
     __shared__ double red[16];
     const RedOp op = ops[blockIdx.x];
     const int nt = blockDim.x;
@@ -465,7 +466,7 @@ __global__ void __launch_bounds__(256) k_reduce(const RedOp* __restrict__ ops, i
         if (i + k * nt < op.len) a[k] += p[i + k * nt];
     const double s = ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
     const double tot = block_sum(s, red);
-    if (threadIdx.x == 0) out[op.dst] = tot;
+    if (threadIdx.x == 0) (op.to_tmp ? tmp : out)[op.dst] = tot;
 }
 
 // ---------------------------------------------------------------------------
@@ -573,9 +574,9 @@ hipError_t launch_fused(bool grad, int S, const ItemSched* items, int nitems, in
 }
 
 hipError_t launch_reduce(const RedOp* ops, int nops, const double* parts, double* out,
-                         hipStream_t st) {
+                         double* tmp, hipStream_t st) {
     if (nops <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_reduce, dim3(nops), dim3(256), 0, st, ops, nops, parts, out);
+    hipLaunchKernelGGL(k_reduce, dim3(nops), dim3(256), 0, st, ops, nops, parts, out, tmp);
     return hipGetLastError();
 }
 
