@@ -69,6 +69,9 @@ def bench_batched(data, steps, warmup, sample_trees, schedule):
     b.set_schedule(schedule)
     for i, m in enumerate(models):
         b.set_model(i, mcp.Model(*m))
+    b.logl_grad(blv)
+    tiles, tune_rows = b.autotune(reps=2)
+    log(f"  autotuned tiles (columns per thread by state count): {tiles}")
     n_items, total_patterns = b.sizes()
     for _ in range(warmup):
         b.logl_grad(blv)
@@ -85,6 +88,8 @@ def bench_batched(data, steps, warmup, sample_trees, schedule):
     fwd_ms, adj_ms = float(np.sum(ms[ph == 2])), float(np.sum(ms[ph == 3]))
     out = {"items": n_items, "patterns_per_step": total_patterns,
            "schedule": schedule,
+           "autotuned_cols_per_thread": {str(k): v for k, v in tiles.items()},
+           "autotune_ms": [list(r) for r in tune_rows],
            "ms_per_step": 1e3 * wall / steps,
            "value": total_patterns * steps / wall,
            "algorithmic_GB_per_step": (bytes_fwd + bytes_adj) / 1e9,
@@ -439,6 +444,9 @@ def main():
     plan = mcp.Plan(w.ta.parent, w.x, len(w.rates), weights=w.weights)
     plan.set_model(mcp.Model(w.U, w.D, w.Uinv, w.mu, w.rates, w.pi))
     plan.set_schedule(args.schedule)
+    plan.logl_grad(blv)
+    tiles, tune_rows = plan.autotune(reps=5)
+    log(f"autotuned tiles (columns per thread by state count): {tiles}")
     for _ in range(args.warmup):
         plan.logl_grad(blv)
     log(f"warmup done ({args.warmup}); timing {args.steps} steps ({args.schedule} schedule)")
@@ -522,6 +530,8 @@ def main():
                      "kernel_ms_per_step": kms,
                      "avg_launch_us": 1e3 * kms / max(nl, 1)},
         "schedule": args.schedule,
+        "autotuned_cols_per_thread": {str(k): v for k, v in tiles.items()},
+        "autotune_ms": [list(r) for r in tune_rows],
         "graph_value": gvalue,
         "instrumented_ms_per_step": 1e3 * iwall / args.steps,
         "kernel_ms_per_step": {"P_build": per_phase[1], "post_order": per_phase[2],

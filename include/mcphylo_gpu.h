@@ -177,6 +177,25 @@ int mcp_use_graph(mcp_batch* batch, int32_t on);
 #define MCP_SCHEDULE_LEVELS 1
 int mcp_set_schedule(mcp_batch* batch, int32_t mode);
 
+/* Tile width of the level kernels for state count S: a task is one
+ * wavefront over 64 * cpt consecutive patterns (cpt = columns per lane).
+ * Built candidates: cpt 1, 2, 4 for S = 2..4 (1 or 2 otherwise); for S = 20,
+ * cpt 0 selects the FP64-MFMA kernels and 1 the generic ones.  cpt < 0
+ * restores the built default.  The choice changes the summation order of the
+ * partial sums (results move in the last bits), so a stored choice can be
+ * reapplied with mcp_set_tiles to reproduce results bit for bit.  No
+ * reference counterpart (the reference threads its loops with @tturbo). */
+int mcp_set_tiles(mcp_batch* batch, int32_t S, int32_t cpt);
+int mcp_get_tiles(mcp_batch* batch, int32_t S, int32_t* cpt);
+
+/* Autotuner: for every state count in the engine, time `reps` full logL +
+ * gradient evaluations (median, after one untimed) with each built tile
+ * width on the engine's own trees and data, and keep the fastest.  Branch
+ * lengths: those of the last evaluation (0.1 everywhere before the first).
+ * report (may be NULL): up to cap rows (S, cpt, ms); *n_report: rows tried.
+ * The model of every item must be set. */
+int mcp_autotune(mcp_batch* batch, int32_t reps, double* report, int32_t cap, int32_t* n_report);
+
 /* Time later evaluations with HIP events on the engine's stream (on != 0).
  * ms[0] whole sequence (first upload to last download); ms[1..4] summed
  * kernel durations of the P/dP build, post-order levels, pre-order levels

@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -26,11 +27,14 @@
 namespace mcp {
 hipError_t launch_build_P(const ModelDev*, const int32_t*, int, long long, hipStream_t);
 hipError_t launch_tip_tables(const ModelDev*, const int32_t*, int, long long, hipStream_t);
-hipError_t launch_level(bool, int, const LevelJob*, const int32_t*, const int32_t*, int, int, int,
-                        double*, hipStream_t);
-hipError_t launch_fused(bool, int, const ItemSched*, int, int, int, const LevelJob*, const int32_t*,
-                        const LevelJob*, const int32_t*, double*, hipStream_t);
-int cols_per_thread(int S);
+hipError_t launch_level(bool, int, int, const LevelJob*, const int32_t*, const int32_t*, int, int,
+                        int, double*, hipStream_t);
+hipError_t launch_fused(bool, int, int, const ItemSched*, int, int, int, const LevelJob*,
+                        const int32_t*, const LevelJob*, const int32_t*, double*, hipStream_t);
+int default_cpt(int S);
+bool cpt_available(int S, int cpt);
+int max_tile_cols(int S);
+int tile_cols(int cpt);
 hipError_t launch_reduce(const RedOp*, int, const double*, double*, double*, hipStream_t);
 bool supported_states(int S);
 hipError_t launch_lf_pre(const HmcItem*, int, double*, double*, const double*, const double*,
@@ -187,7 +191,7 @@ struct ItemHost {
 
 struct Launch {
     bool adjoint;
-    int S;
+    int S, cpt;
     int job_begin, njobs;
     int taskoff_begin;  // index into task_off arena (njobs+1 entries)
     int ntasks;
@@ -214,8 +218,14 @@ struct mcp_engine {
     std::vector<Launch> fwd, adj;
     std::vector<int32_t> taskoff, blkjob;
     struct FusedLaunch {
-        int S, sched_begin, nitems, nblocks;
+        int S, cpt, sched_begin, nitems, nblocks;
     };
+    // columns per thread chosen for each state count (-1: the built default);
+    // set by mcp_set_tiles / mcp_autotune
+    int cpt_sel[33] = {-1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1,
+                       -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1};
+    int cpt(int S) const { return (S < 33 && cpt_sel[S] >= 0) ? cpt_sel[S] : default_cpt(S); }
+    bool evaluated = false;  // branch lengths uploaded at least once
     std::vector<FusedLaunch> fused;
     // reductions: stage 1 over the partial sums (ll ops first, then gradient
     // ops; long ones split into chunks into a temporary), stage 2 combining
@@ -367,7 +377,7 @@ int build_dataset(const mcp_dataset_desc& d, DatasetHost& out) {
         w = wu;
     }
     out.C = Cu;
-    out.Cp = (int)round_up((size_t)Cu, (size_t)64 * cols_per_thread(S));
+    out.Cp = (int)round_up((size_t)Cu, (size_t)max_tile_cols(S));
     out.codes.assign((size_t)L * out.Cp, 0);
     for (int j = 0; j < L; ++j)
         for (int c = 0; c < Cu; ++c)
@@ -380,6 +390,8 @@ int build_dataset(const mcp_dataset_desc& d, DatasetHost& out) {
 // ---------------------------------------------------------------------------
 // items: topology tables
 // ---------------------------------------------------------------------------
+
+void assign_partial_slots(ItemHost& it);
 
 int build_topology(ItemHost& it, const DatasetHost& d, const int32_t* parent) {
     const int N = it.N;
@@ -467,15 +479,23 @@ int build_topology(ItemHost& it, const DatasetHost& d, const int32_t* parent) {
     it.g_slots[0] = it.g_slots[1] = 0;
     for (int dd = 0; dd <= it.max_depth; ++dd)
         it.g_slots[dd & 1] = std::max(it.g_slots[dd & 1], gcount[dd]);
-    // partial-sum offsets (relative to the item; made absolute in layout)
+    (void)fslot;
+    (void)lidx;
+    assign_partial_slots(it);
+    return MCP_OK;
+}
+
+// Partial-sum slots of every node relative to the item (made absolute in the
+// layout): R * ntiles per internal node (log scalers), then per branch
+// (gradient); they follow the item's tile count.
+void assign_partial_slots(ItemHost& it) {
     const int per = it.R * it.ntiles;
-    for (int v = 0; v < N; ++v) {
+    int lidx = 0;
+    for (int v = 0; v < it.N; ++v) {
         NodeInfo& n = it.nodes[v];
         if (n.nchild > 0) n.lpart = (lidx++) * per;
     }
-    for (int v = 0; v < N - 1; ++v) it.nodes[v].gpart = (it.nint + v) * per;
-    (void)fslot;
-    return MCP_OK;
+    for (int v = 0; v < it.N - 1; ++v) it.nodes[v].gpart = (it.nint + v) * per;
 }
 
 int item_from_desc(const mcp_item_desc& d, const std::vector<DatasetHost>& ds, ItemHost& it) {
@@ -488,7 +508,7 @@ int item_from_desc(const mcp_item_desc& d, const std::vector<DatasetHost>& ds, I
     it.N = d.n_nodes;
     it.R = d.n_rates;
     it.S = D.S;
-    it.ntiles = D.Cp / (64 * cols_per_thread(D.S));
+    it.ntiles = D.Cp / tile_cols(default_cpt(D.S));
     return build_topology(it, D, d.parent);
 }
 
@@ -687,6 +707,7 @@ static int rebuild_schedule(mcp_engine* e) {
         Launch L;
         L.adjoint = adjoint;
         L.S = S;
+        L.cpt = e->cpt(S);
         L.job_begin = (int)js.size();
         L.taskoff_begin = (int)e->taskoff.size();
         long long tasks = 0;
@@ -768,6 +789,7 @@ static int rebuild_schedule(mcp_engine* e) {
         for (size_t k = 0; k < svals.size(); ++k) {
             mcp_engine::FusedLaunch F{};
             F.S = svals[k];
+            F.cpt = e->cpt(svals[k]);
             F.sched_begin = (int)sched.size();
             long long blocks = 0;
             for (ItemSched I : per_s[k]) {
@@ -918,6 +940,27 @@ static int engine_create(mcp_engine** out, const mcp_batch_desc* d) {
     return MCP_OK;
 }
 
+// Tile widths changed: every item's tile count, the partial-sum slots and
+// the schedule follow (device slabs keep their layout: the pattern axis is
+// padded to the widest candidate).
+static int relayout_tiles(mcp_engine* e) {
+    long long parts = 0;
+    for (auto& it : e->items) {
+        const DatasetHost& D = e->ds[it.dataset];
+        it.ntiles = D.Cp / tile_cols(e->cpt(it.S));
+        assign_partial_slots(it);
+        const long long nparts = (long long)(it.nint + it.N - 1) * it.R * it.ntiles;
+        if (parts + nparts > 0x7fffffffLL)
+            return fail(MCP_ERR_UNSUPPORTED, "batch too large for 32-bit partial-sum offsets");
+        it.parts_off = (int)parts;
+        parts += nparts;
+    }
+    int st;
+    if ((st = e->dparts.ensure((size_t)std::max<long long>(parts, 1) * sizeof(double)))) return st;
+    HIP_TRY(hipMemset(e->dparts.p, 0, (size_t)std::max<long long>(parts, 1) * sizeof(double)));
+    return rebuild_schedule(e);
+}
+
 static int engine_set_model(mcp_engine* e, int item, const double* U, const double* D,
                             const double* Uinv, double mu, const double* rates, const double* pi) {
     if (!e) return fail(MCP_ERR_INVALID, "null handle");
@@ -1014,21 +1057,21 @@ static int enqueue_eval(mcp_engine* e, bool grad) {
     TIMED(1, launch_tip_tables(e->dmodels.as<ModelDev>(), e->delemT.as<int32_t>(),
                                (int)e->items.size(), e->nelemT, st));
     for (const Launch& L : e->fwd) {
-        TIMED(2, launch_level(false, L.S, e->djobs_f.as<LevelJob>() + L.job_begin,
+        TIMED(2, launch_level(false, L.S, L.cpt, e->djobs_f.as<LevelJob>() + L.job_begin,
                               e->dtaskoff.as<int32_t>() + L.taskoff_begin,
                               e->dblkjob.as<int32_t>() + L.blk_begin, L.ntasks, L.chunk, L.grid,
                               e->dparts.as<double>(), st));
     }
     if (grad)
         for (const Launch& L : e->adj) {
-            TIMED(3, launch_level(true, L.S, e->djobs_a.as<LevelJob>() + L.job_begin,
+            TIMED(3, launch_level(true, L.S, L.cpt, e->djobs_a.as<LevelJob>() + L.job_begin,
                                   e->dtaskoff.as<int32_t>() + L.taskoff_begin,
                                   e->dblkjob.as<int32_t>() + L.blk_begin, L.ntasks, L.chunk,
                                   L.grid, e->dparts.as<double>(), st));
         }
     for (const auto& F : e->fused) {
         TIMED(grad ? 3 : 2,
-              launch_fused(grad, F.S, e->dsched.as<ItemSched>() + F.sched_begin, F.nitems,
+              launch_fused(grad, F.S, F.cpt, e->dsched.as<ItemSched>() + F.sched_begin, F.nitems,
                            F.nblocks, e->waves, e->djobs_f.as<LevelJob>(), e->dlvl_f.as<int32_t>(),
                            e->djobs_a.as<LevelJob>(), e->dlvl_a.as<int32_t>(),
                            e->dparts.as<double>(), st));
@@ -1054,6 +1097,7 @@ static int engine_eval(mcp_engine* e, const double* blv, double* ll, double* gra
         if (!it.model_set) return fail(MCP_ERR_STATE, "substitution model not set");
     DeviceGuard guard(e->device);
     std::memcpy(e->hblv.p, blv, e->total_blv * sizeof(double));
+    e->evaluated = true;
     const int gi = want_grad ? 1 : 0;
     if (e->use_graph) {
         if (!e->graph_exec[gi]) {
@@ -1151,17 +1195,17 @@ static int enqueue_leapfrog(mcp_engine* e) {
     HIP_TRY(launch_tip_tables(e->dmodels.as<ModelDev>(), e->delemT.as<int32_t>(), n, e->nelemT,
                               st));
     for (const Launch& L : e->fwd)
-        HIP_TRY(launch_level(false, L.S, e->djobs_f.as<LevelJob>() + L.job_begin,
+        HIP_TRY(launch_level(false, L.S, L.cpt, e->djobs_f.as<LevelJob>() + L.job_begin,
                              e->dtaskoff.as<int32_t>() + L.taskoff_begin,
                              e->dblkjob.as<int32_t>() + L.blk_begin, L.ntasks, L.chunk, L.grid,
                              e->dparts.as<double>(), st));
     for (const Launch& L : e->adj)
-        HIP_TRY(launch_level(true, L.S, e->djobs_a.as<LevelJob>() + L.job_begin,
+        HIP_TRY(launch_level(true, L.S, L.cpt, e->djobs_a.as<LevelJob>() + L.job_begin,
                              e->dtaskoff.as<int32_t>() + L.taskoff_begin,
                              e->dblkjob.as<int32_t>() + L.blk_begin, L.ntasks, L.chunk, L.grid,
                              e->dparts.as<double>(), st));
     for (const auto& F : e->fused)
-        HIP_TRY(launch_fused(true, F.S, e->dsched.as<ItemSched>() + F.sched_begin, F.nitems,
+        HIP_TRY(launch_fused(true, F.S, F.cpt, e->dsched.as<ItemSched>() + F.sched_begin, F.nitems,
                              F.nblocks, e->waves, e->djobs_f.as<LevelJob>(),
                              e->dlvl_f.as<int32_t>(), e->djobs_a.as<LevelJob>(),
                              e->dlvl_a.as<int32_t>(), e->dparts.as<double>(), st));
@@ -1362,6 +1406,86 @@ int mcp_set_schedule(mcp_batch* b, int32_t mode) {
     if (b->schedule == mode) return MCP_OK;
     b->schedule = mode;
     return rebuild_schedule(b);
+}
+
+int mcp_set_tiles(mcp_batch* b, int32_t S, int32_t cpt) {
+    if (!b) return fail(MCP_ERR_INVALID, "null handle");
+    if (!supported_states(S)) return fail(MCP_ERR_UNSUPPORTED, "no kernels for this state count");
+    if (cpt < 0) cpt = default_cpt(S);
+    if (!cpt_available(S, cpt))
+        return fail(MCP_ERR_UNSUPPORTED, "no kernels for this (state count, columns per thread)");
+    DeviceGuard guard(b->device);
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    if (b->cpt(S) == cpt) return MCP_OK;
+    b->cpt_sel[S] = cpt;
+    return relayout_tiles(b);
+}
+
+int mcp_get_tiles(mcp_batch* b, int32_t S, int32_t* cpt) {
+    if (!b || !cpt) return fail(MCP_ERR_INVALID, "null argument");
+    if (!supported_states(S)) return fail(MCP_ERR_UNSUPPORTED, "no kernels for this state count");
+    *cpt = b->cpt(S);
+    return MCP_OK;
+}
+
+int mcp_autotune(mcp_batch* b, int32_t reps, double* report, int32_t cap, int32_t* n_report) {
+    if (!b) return fail(MCP_ERR_INVALID, "null handle");
+    if (reps < 1 || reps > 1000) return fail(MCP_ERR_INVALID, "reps must be in 1..1000");
+    for (auto& it : b->items)
+        if (!it.model_set) return fail(MCP_ERR_STATE, "substitution model not set");
+    DeviceGuard guard(b->device);
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    std::vector<double> blv((size_t)b->total_blv, 0.1);
+    if (b->evaluated) std::memcpy(blv.data(), b->hblv.p, blv.size() * sizeof(double));
+    std::vector<double> ll(b->items.size()), grad((size_t)b->total_blv);
+    std::vector<int> svals;
+    for (auto& it : b->items)
+        if (std::find(svals.begin(), svals.end(), it.S) == svals.end()) svals.push_back(it.S);
+    std::sort(svals.begin(), svals.end());
+    const bool graph = b->use_graph, timing = b->timing;
+    b->use_graph = false;
+    b->timing = false;
+    int nrep = 0, st = MCP_OK;
+    for (int S : svals) {
+        std::vector<int> cands;
+        for (int c = 0; c <= 8; ++c)
+            if (cpt_available(S, c)) cands.push_back(c);
+        int best = b->cpt(S);
+        double best_ms = 1e300;
+        for (int c : cands) {
+            b->cpt_sel[S] = c;
+            if ((st = relayout_tiles(b))) break;
+            // one untimed evaluation, then the median of reps timed ones
+            if ((st = engine_eval(b, blv.data(), ll.data(), grad.data(), true))) break;
+            std::vector<double> t(reps);
+            for (int k = 0; k < reps && !st; ++k) {
+                const auto t0 = std::chrono::steady_clock::now();
+                st = engine_eval(b, blv.data(), ll.data(), grad.data(), true);
+                t[k] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0)
+                           .count();
+            }
+            if (st) break;
+            std::sort(t.begin(), t.end());
+            const double ms = t[reps / 2];
+            if (report && nrep < cap) {
+                report[3 * nrep] = S;
+                report[3 * nrep + 1] = c;
+                report[3 * nrep + 2] = ms;
+            }
+            ++nrep;
+            if (ms < best_ms) {
+                best_ms = ms;
+                best = c;
+            }
+        }
+        if (st) break;
+        b->cpt_sel[S] = best;
+    }
+    const int st2 = relayout_tiles(b);
+    b->use_graph = graph;
+    b->timing = timing;
+    if (n_report) *n_report = nrep;
+    return st ? st : st2;
 }
 
 int mcp_set_timing(mcp_batch* b, int32_t on) {

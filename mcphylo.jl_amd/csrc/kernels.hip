@@ -491,7 +491,8 @@ hipError_t launch_tip_tables(const ModelDev* models, const int32_t* elem_off, in
     return hipGetLastError();
 }
 
-// State counts with built kernels and the columns per thread used for each.
+// State counts with built kernels and their default columns per thread (CPT:
+// a task is one wavefront over a tile of 64 * CPT consecutive patterns).
 #ifndef MCP_CPT_S2
 #define MCP_CPT_S2 4
 #endif
@@ -502,6 +503,15 @@ hipError_t launch_tip_tables(const ModelDev* models, const int32_t* elem_off, in
     X(1, 2) X(2, MCP_CPT_S2) X(3, 2) X(4, MCP_CPT_S4) X(5, 1) X(6, 1) X(7, 1) X(8, 1) X(16, 1) \
     X(20, 1)
 
+// Every (S, CPT) pair with built kernels: the candidates of the tile autotuner.
+#define MCP_FOR_EACH_SC(X)                                                                   \
+    X(1, 2) X(2, 1) X(2, 2) X(2, 4) X(3, 1) X(3, 2) X(3, 4) X(4, 1) X(4, 2) X(4, 4) X(5, 1) \
+    X(6, 1) X(7, 1) X(8, 1) X(16, 1) X(20, 1)
+
+#ifndef MCP_MFMA20
+#define MCP_MFMA20 1
+#endif
+
 bool supported_states(int S) {
     switch (S) {
 #define X(n, c) case n: return true;
@@ -511,7 +521,10 @@ bool supported_states(int S) {
     }
 }
 
-int cols_per_thread(int S) {
+// Default CPT of S; for S = 20, 0 selects the FP64-MFMA level kernels (their
+// tile is 64 patterns, like CPT 1).
+int default_cpt(int S) {
+    if (S == 20) return MCP_MFMA20 ? 0 : 1;
     switch (S) {
 #define X(n, c) case n: return c;
         MCP_FOR_EACH_S(X)
@@ -520,57 +533,70 @@ int cols_per_thread(int S) {
     }
 }
 
+bool cpt_available(int S, int cpt) {
+    if (S == 20 && cpt == 0) return true;
+#define X(n, c) if (S == n && cpt == c) return true;
+    MCP_FOR_EACH_SC(X)
+#undef X
+    return false;
+}
+
+// Widest tile over the candidates of S: the pattern axis is padded to it so
+// that every candidate fits the same layout.
+int max_tile_cols(int S) {
+    int m = 1;
+#define X(n, c) if (S == n && c > m) m = c;
+    MCP_FOR_EACH_SC(X)
+#undef X
+    return 64 * m;
+}
+
+int tile_cols(int cpt) { return 64 * (cpt < 1 ? 1 : cpt); }
+
 hipError_t launch_level20(bool adjoint, const LevelJob* jobs, const int32_t* task_off,
                           const int32_t* blk_job, int total, int chunk, int grid, double* parts,
                           hipStream_t st);
 
-#ifndef MCP_MFMA20
-#define MCP_MFMA20 1
-#endif
-
-hipError_t launch_level(bool adjoint, int S, const LevelJob* jobs, const int32_t* task_off,
-                        const int32_t* blk_job, int total, int chunk, int grid, double* parts,
-                        hipStream_t st) {
+hipError_t launch_level(bool adjoint, int S, int cpt, const LevelJob* jobs,
+                        const int32_t* task_off, const int32_t* blk_job, int total, int chunk,
+                        int grid, double* parts, hipStream_t st) {
     if (total <= 0 || grid <= 0) return hipSuccess;
-    if (S == 20 && MCP_MFMA20)
+    if (S == 20 && cpt == 0)
         return launch_level20(adjoint, jobs, task_off, blk_job, total, chunk, grid, parts, st);
-    switch (S) {
 #define X(n, c)                                                                                  \
-    case n:                                                                                      \
+    if (S == n && cpt == c) {                                                                    \
         if (adjoint)                                                                             \
             hipLaunchKernelGGL((k_level<n, c, true>), dim3(grid), dim3(64), 0, st, jobs,         \
                                task_off, blk_job, total, chunk, parts);                          \
         else                                                                                     \
             hipLaunchKernelGGL((k_level<n, c, false>), dim3(grid), dim3(64), 0, st, jobs,        \
                                task_off, blk_job, total, chunk, parts);                          \
-        break;
-        MCP_FOR_EACH_S(X)
-#undef X
-        default: return hipErrorInvalidValue;
+        return hipGetLastError();                                                                \
     }
-    return hipGetLastError();
+    MCP_FOR_EACH_SC(X)
+#undef X
+    return hipErrorInvalidValue;
 }
 
-hipError_t launch_fused(bool grad, int S, const ItemSched* items, int nitems, int nblocks,
-                        int waves, const LevelJob* jobs_f, const int32_t* lvl_f,
+hipError_t launch_fused(bool grad, int S, int cpt, const ItemSched* items, int nitems,
+                        int nblocks, int waves, const LevelJob* jobs_f, const int32_t* lvl_f,
                         const LevelJob* jobs_a, const int32_t* lvl_a, double* parts,
                         hipStream_t st) {
     if (nitems <= 0 || nblocks <= 0) return hipSuccess;
-    switch (S) {
+    if (cpt == 0) cpt = 1;  // the fused schedule has no MFMA variant; same 64-pattern tile
 #define X(n, c)                                                                                  \
-    case n:                                                                                      \
+    if (S == n && cpt == c) {                                                                    \
         if (grad)                                                                                \
             hipLaunchKernelGGL((k_fused<n, c, true>), dim3(nblocks), dim3(64 * waves), 0, st,    \
                                items, nitems, jobs_f, lvl_f, jobs_a, lvl_a, parts);              \
         else                                                                                     \
             hipLaunchKernelGGL((k_fused<n, c, false>), dim3(nblocks), dim3(64 * waves), 0, st,   \
                                items, nitems, jobs_f, lvl_f, jobs_a, lvl_a, parts);              \
-        break;
-        MCP_FOR_EACH_S(X)
-#undef X
-        default: return hipErrorInvalidValue;
+        return hipGetLastError();                                                                \
     }
-    return hipGetLastError();
+    MCP_FOR_EACH_SC(X)
+#undef X
+    return hipErrorInvalidValue;
 }
 
 hipError_t launch_reduce(const RedOp* ops, int nops, const double* parts, double* out,

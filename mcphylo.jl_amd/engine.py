@@ -65,6 +65,28 @@ class _Handle:
         m = {"fused": _lib.MCP_SCHEDULE_FUSED, "levels": _lib.MCP_SCHEDULE_LEVELS}[mode]
         check(_lib.load_library().mcp_set_schedule(self._h, m))
 
+    def set_tiles(self, S: int, cpt: int) -> None:
+        """Columns per thread of the level kernels for state count S (-1:
+        built default; for S = 20, 0 = FP64-MFMA kernels)."""
+        check(_lib.load_library().mcp_set_tiles(self._h, int(S), int(cpt)))
+
+    def tiles(self, S: int) -> int:
+        out = np.zeros(1, dtype=np.int32)
+        check(_lib.load_library().mcp_get_tiles(self._h, int(S), iptr(out)))
+        return int(out[0])
+
+    def autotune(self, reps: int = 5):
+        """Time every built tile width per state count on this engine's data
+        and keep the fastest.  Returns ({S: cpt}, [(S, cpt, ms), ...])."""
+        cap = 64
+        rep = np.zeros(3 * cap)
+        n = np.zeros(1, dtype=np.int32)
+        check(_lib.load_library().mcp_autotune(self._h, int(reps), dptr(rep), cap, iptr(n)))
+        rows = [(int(rep[3 * k]), int(rep[3 * k + 1]), float(rep[3 * k + 2]))
+                for k in range(min(int(n[0]), cap))]
+        chosen = {S: self.tiles(S) for S in sorted({r[0] for r in rows})}
+        return chosen, rows
+
     def set_timing(self, on: bool = True) -> None:
         check(_lib.load_library().mcp_set_timing(self._h, 1 if on else 0))
 

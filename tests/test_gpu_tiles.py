@@ -1,0 +1,75 @@
+"""GPU: every built tile width (columns per thread) of the level and fused
+kernels against the CPU oracle, and the autotuner on a mixed batch."""
+
+import numpy as np
+import pytest
+
+from conftest import mcp, oracle, rel_err
+from test_gpu_parity import GRAD_TOL, LL_TOL, grad_err
+
+pytestmark = pytest.mark.gpu
+
+CANDIDATES = {2: [1, 2, 4], 3: [1, 2, 4], 4: [1, 2, 4], 20: [0, 1]}
+
+
+def _case(seed, n, S, C, R):
+    w = mcp.synthetic.make_workload("custom", seed=seed, n=n, S=S, C=C, R=R)
+    blv = mcp.get_branchlength_vector(w.tree)
+    llo, go = oracle.logl_grad(w.ta, w.x, w.U, w.D, w.Uinv, w.mu, w.rates, w.pi, blv)
+    return w, blv, llo, go
+
+
+@pytest.mark.parametrize("seed,n,S,C,R", [(501, 23, 2, 700, 4), (502, 19, 3, 300, 2),
+                                          (503, 31, 4, 1100, 4), (504, 12, 20, 300, 2)])
+@pytest.mark.parametrize("schedule", ["levels", "fused"])
+def test_every_tile_width_matches_oracle(seed, n, S, C, R, schedule):
+    print(f"tiles seed={seed} S={S} C={C} R={R} {schedule}", flush=True)
+    w, blv, llo, go = _case(seed, n, S, C, R)
+    p = mcp.Plan(w.ta.parent, w.x, R)
+    p.set_model(mcp.Model(w.U, w.D, w.Uinv, w.mu, w.rates, w.pi))
+    p.set_schedule(schedule)
+    for cpt in CANDIDATES[S]:
+        p.set_tiles(S, cpt)
+        assert p.tiles(S) == cpt
+        ll, g = p.logl_grad(blv)
+        assert rel_err(ll, llo) < LL_TOL, cpt
+        assert grad_err(g, go) < GRAD_TOL, cpt
+        assert rel_err(p.logl(blv), llo) < LL_TOL, cpt
+    p.set_tiles(S, -1)
+    with pytest.raises(mcp.McpArgumentError):
+        p.set_tiles(S, 3)
+    p.close()
+
+
+def test_autotune_mixed_batch_keeps_results_and_reports_candidates():
+    data = mcp.synthetic.make_batched_workload(n_trees=6, chains=2, n_range=(16, 60),
+                                               c_range=(300, 2000))
+    ds, items, models, blv = data
+    b = mcp.Batch(ds, items)
+    for i, m in enumerate(models):
+        b.set_model(i, mcp.Model(*m))
+    ll0, g0 = b.logl_grad(blv)
+    chosen, rows = b.autotune(reps=2)
+    S_present = sorted({b.item_S[i] for i in range(len(items))})
+    assert sorted(chosen) == S_present
+    assert len(rows) == sum(len(CANDIDATES[S]) for S in S_present)
+    for S, cpt in chosen.items():
+        best = min((r for r in rows if r[0] == S), key=lambda r: r[2])
+        assert cpt == best[1]
+    ll1, g1 = b.logl_grad(blv)
+    assert np.max(np.abs(ll1 - ll0) / np.abs(ll0)) < 1e-12
+    assert np.max(np.abs(g1 - g0)) <= 1e-10 * np.max(np.abs(g0))
+    # a stored choice reproduces the tuned results bit for bit in a new engine
+    b2 = mcp.Batch(ds, items)
+    for i, m in enumerate(models):
+        b2.set_model(i, mcp.Model(*m))
+    for S, cpt in chosen.items():
+        b2.set_tiles(S, cpt)
+    ll2, g2 = b2.logl_grad(blv)
+    assert np.array_equal(ll1, ll2) and np.array_equal(g1, g2)
+    # graph replay after tuning
+    b.use_graph(True)
+    ll3, g3 = b.logl_grad(blv)
+    assert np.array_equal(ll3, ll1) and np.array_equal(g3, g1)
+    b.close()
+    b2.close()
