@@ -146,6 +146,17 @@ int mcp_batch_logl_grad(mcp_batch* batch, const double* blv_concat, double* ll,
  * Exponential with mean `scale` (src/Likelihood/Prior.jl:50-56,80-83). */
 int mcp_hmc_set_prior(mcp_batch* batch, int32_t item, int32_t kind, double scale);
 
+/* General form: kind 0 none (nparams 0); kind 1 exponentialBL
+ * (params = {scale}); kind 2 CompoundDirichlet (params = {alpha, a, beta,
+ * c}, src/Likelihood/Prior.jl:1-37 with the field order of
+ * CompoundDirichlet(alpha, a, beta, c), src/distributions/TreeDistribution.jl:23-39):
+ *   logprior = alpha log beta - lgamma(alpha) - beta T - lgamma(a) - lgamma(c)
+ *              + lgamma(a + c) + (a - 1) sum_leaf log t + (a c - 1) sum_int log t
+ *              + (alpha - a n_leaf - a c (n_leaf - 3)) log T,   T = sum t,
+ * where leaf branches are those above leaves and internal the others. */
+int mcp_hmc_set_prior_params(mcp_batch* batch, int32_t item, int32_t kind, const double* params,
+                             int32_t nparams);
+
 /* One leapfrog step for every item, as refraction! does it
  * (src/samplers/tree_hamiltonian/refraction.jl:2-31; reflection at zero in
  * time order as in ref_NNI!, :35-91, without the NNI topology attempt):

@@ -23,7 +23,7 @@ from .nexus import ParseNexus, compress_patterns, datafortree
 from .phylodist import (MultiplePhyloDist, PhyloDist, __logpdf, clear_plan_cache, gradlogpdf,
                         logpdf, maximum, minimum, size)
 from .rates import discrete_gamma_rates, mean_boundaries, median_boundaries
-from . import sampler, surrogate, synthetic
+from . import priors, sampler, surrogate, synthetic
 from .surrogate import Surrogate
 from .tree import (Node, ParseNewick, TreeArrays, get_branchlength_vector, get_leaves, get_mother,
                    number_nodes, post_order, pre_order, set_branchlength_vector, to_newick,

@@ -35,7 +35,8 @@ EXPORTED = (
     "mcp_plan_create", "mcp_plan_destroy", "mcp_set_model", "mcp_set_topology", "mcp_logl",
     "mcp_logl_grad", "mcp_batch_create", "mcp_batch_destroy", "mcp_batch_set_model",
     "mcp_batch_set_topology", "mcp_batch_logl", "mcp_batch_logl_grad", "mcp_use_graph",
-    "mcp_set_schedule", "mcp_hmc_set_prior", "mcp_hmc_leapfrog", "mcp_set_tiles", "mcp_get_tiles",
+    "mcp_set_schedule", "mcp_hmc_set_prior", "mcp_hmc_set_prior_params", "mcp_hmc_leapfrog",
+    "mcp_set_tiles", "mcp_get_tiles",
     "mcp_autotune",
     "mcp_set_timing", "mcp_get_timing", "mcp_get_launch_times", "mcp_get_bytes", "mcp_get_sizes",
     "mcp_mlp_create", "mcp_mlp_destroy", "mcp_mlp_layout", "mcp_mlp_get_params",
@@ -137,6 +138,8 @@ def load_library():
         "mcp_use_graph": (ctypes.c_int, [vp, ctypes.c_int32]),
         "mcp_set_schedule": (ctypes.c_int, [vp, ctypes.c_int32]),
         "mcp_hmc_set_prior": (ctypes.c_int, [vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_double]),
+        "mcp_hmc_set_prior_params": (ctypes.c_int, [vp, ctypes.c_int32, ctypes.c_int32, _dp,
+                                                     ctypes.c_int32]),
         "mcp_hmc_leapfrog": (ctypes.c_int, [vp, _dp, _dp, _dp, _dp, _dp, _dp]),
         "mcp_set_tiles": (ctypes.c_int, [vp, ctypes.c_int32, ctypes.c_int32]),
         "mcp_get_tiles": (ctypes.c_int, [vp, ctypes.c_int32, _ip]),

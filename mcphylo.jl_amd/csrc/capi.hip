@@ -40,7 +40,7 @@ bool supported_states(int S);
 hipError_t launch_lf_pre(const HmcItem*, int, double*, double*, const double*, const double*,
                          double*, int, hipStream_t);
 hipError_t launch_lf_post(const HmcItem*, int, const double*, double*, double*, const double*,
-                          const double*, double*, hipStream_t);
+                          const double*, double*, const uint8_t*, hipStream_t);
 }  // namespace mcp
 
 using namespace mcp;
@@ -240,7 +240,7 @@ struct mcp_engine {
     bool use_graph = false;
     hipGraphExec_t graph_exec[3] = {nullptr, nullptr, nullptr};  // logl, logl+grad, leapfrog
     // device leapfrog: [x | r | g] per item (blv layout), eps/delta pairs, log posterior
-    DevBuf dhmc, dhmc_items, dlp;
+    DevBuf dhmc, dhmc_items, dlp, dhmc_int;  // dhmc_int: 1 per internal branch (blv layout)
     PinnedBuf hhmc;
     std::vector<HmcItem> hmc_items;
     bool hmc_items_dirty = true;
@@ -1168,6 +1168,15 @@ static int hmc_prepare(mcp_engine* e) {
     int st;
     if ((st = upload(e, e->dhmc_items, e->hmc_items.data(), n * sizeof(HmcItem)))) return st;
     const size_t T = e->total_blv;
+    {
+        std::vector<uint8_t> inl(std::max<size_t>(T, 1), 0);
+        for (int i = 0; i < n; ++i) {
+            const ItemHost& it = e->items[i];
+            for (int v = 0; v < it.N - 1; ++v)
+                inl[it.blv_off + v] = it.nodes[v].nchild > 0 ? 1 : 0;
+        }
+        if ((st = upload(e, e->dhmc_int, inl.data(), inl.size()))) return st;
+    }
     if ((st = e->dhmc.ensure((3 * T + 2 * (size_t)n) * sizeof(double)))) return st;
     if ((st = e->dlp.ensure((size_t)n * sizeof(double)))) return st;
     if ((st = e->hhmc.ensure((3 * T + 2 * (size_t)n + n + e->total_out) * sizeof(double)))) return st;
@@ -1214,7 +1223,7 @@ static int enqueue_leapfrog(mcp_engine* e) {
     HIP_TRY(launch_reduce(e->dred2.as<RedOp>(), (int)e->red2.size(), e->dredtmp.as<double>(),
                           e->dout.as<double>(), e->dredtmp.as<double>(), st));
     HIP_TRY(launch_lf_post(e->dhmc_items.as<HmcItem>(), n, dx, dr, dg, ded, e->dout.as<double>(),
-                           e->dlp.as<double>(), st));
+                           e->dlp.as<double>(), e->dhmc_int.as<uint8_t>(), st));
     HIP_TRY(hipMemcpyAsync(h, dx, 3 * T * sizeof(double), hipMemcpyDeviceToHost, st));
     HIP_TRY(hipMemcpyAsync(h + 3 * T + 2 * (size_t)n, e->dlp.p, n * sizeof(double),
                            hipMemcpyDeviceToHost, st));
@@ -1379,6 +1388,30 @@ int mcp_hmc_set_prior(mcp_batch* b, int32_t item, int32_t kind, double scale) {
     b->hmc_items[item].prior = kind;
     b->hmc_items[item].prior_scale = kind == 1 ? scale : 1.0;
     HIP_TRY(hipMemcpy(b->dhmc_items.as<HmcItem>() + item, &b->hmc_items[item], sizeof(HmcItem),
+                      hipMemcpyHostToDevice));
+    return MCP_OK;
+}
+
+int mcp_hmc_set_prior_params(mcp_batch* b, int32_t item, int32_t kind, const double* params,
+                             int32_t nparams) {
+    if (!b) return fail(MCP_ERR_INVALID, "null handle");
+    if (item < 0 || item >= (int32_t)b->items.size()) return fail(MCP_ERR_INVALID, "item out of range");
+    const int need = kind == 0 ? 0 : kind == 1 ? 1 : kind == 2 ? 4 : -1;
+    if (need < 0) return fail(MCP_ERR_INVALID, "unknown prior");
+    if (nparams != need || (need > 0 && !params))
+        return fail(MCP_ERR_INVALID, "wrong number of prior parameters");
+    for (int k = 0; k < need; ++k)
+        if (!(params[k] > 0.0) || !std::isfinite(params[k]))
+            return fail(MCP_ERR_INVALID, "prior parameters must be positive and finite");
+    DeviceGuard guard(b->device);
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    int st = hmc_prepare(b);
+    if (st) return st;
+    HmcItem& h = b->hmc_items[item];
+    h.prior = kind;
+    h.prior_scale = kind == 1 ? params[0] : 1.0;
+    for (int k = 0; k < 4; ++k) h.p[k] = (kind == 2) ? params[k] : 0.0;
+    HIP_TRY(hipMemcpy(b->dhmc_items.as<HmcItem>() + item, &h, sizeof(HmcItem),
                       hipMemcpyHostToDevice));
     return MCP_OK;
 }

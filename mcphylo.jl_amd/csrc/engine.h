@@ -92,12 +92,16 @@ namespace mcp {
 // Per-item record of the device leapfrog (reference: refraction!,
 // src/samplers/tree_hamiltonian/refraction.jl:2-31, without NNI moves).
 // State buffers are packed per item at offset `off` (n = N-1 branches).
+// Branch-length prior: 0 none; 1 exponentialBL(prior_scale)
+// (src/Likelihood/Prior.jl:50-56,80-83); 2 CompoundDirichlet with
+// p = (alpha, a, beta, c) (Prior.jl:1-37).
 struct HmcItem {
     int32_t n;             // branches
     int32_t off;           // offset of this item's branch vectors
     int32_t out;           // offset of (ll, grad) in the engine's output buffer
-    int32_t prior;         // 0: none, 1: exponential(scale) per branch
+    int32_t prior;         // prior kind (above)
     double prior_scale;
+    double p[4];
 };
 
 }  // namespace mcp

@@ -718,14 +718,18 @@ __global__ void __launch_bounds__(256) k_lf_pre(const HmcItem* __restrict__ item
 }
 
 // Second half: g = (dll + dlogprior) * scale_fac(x, delta); r += eps/2 g;
-// lp = ll + logprior(x).
+// lp = ll + logprior(x).  The CompoundDirichlet prior (Prior.jl:1-37) needs
+// block sums first: total length, sums of log lengths over leaf and over
+// internal branches (`internal` flags, blv layout), and the leaf count.
 __global__ void __launch_bounds__(256) k_lf_post(const HmcItem* __restrict__ items,
                                                  const double* __restrict__ x,
                                                  double* __restrict__ r, double* __restrict__ g,
                                                  const double* __restrict__ eps_delta,
                                                  const double* __restrict__ out,
-                                                 double* __restrict__ lp) {
+                                                 double* __restrict__ lp,
+                                                 const uint8_t* __restrict__ internal) {
     __shared__ double red[16];
+    __shared__ double bc[4];
     const HmcItem it = items[blockIdx.x];
     const double eps = eps_delta[2 * blockIdx.x];
     const double delta = eps_delta[2 * blockIdx.x + 1];
@@ -733,19 +737,51 @@ __global__ void __launch_bounds__(256) k_lf_post(const HmcItem* __restrict__ ite
     double* ri = r + it.off;
     double* gi = g + it.off;
     const double* grad = out + it.out + 1;
-    double lpr = 0.0;
+    const uint8_t* inl = internal ? internal + it.off : nullptr;
+    double lpr = 0.0, cd_lp = 0.0, cd_k = 0.0;
+    if (it.prior == 2) {
+        double sx = 0.0, sll = 0.0, sli = 0.0, nl = 0.0;
+        for (int k = threadIdx.x; k < it.n; k += blockDim.x) {
+            const double xk = xi[k], lk = log(xk);
+            sx += xk;
+            if (inl && inl[k]) {
+                sli += lk;
+            } else {
+                sll += lk;
+                nl += 1.0;
+            }
+        }
+        sx = block_sum(sx, red);
+        if (threadIdx.x == 0) bc[0] = sx;
+        sll = block_sum(sll, red);
+        if (threadIdx.x == 0) bc[1] = sll;
+        sli = block_sum(sli, red);
+        if (threadIdx.x == 0) bc[2] = sli;
+        nl = block_sum(nl, red);
+        if (threadIdx.x == 0) bc[3] = nl;
+        __syncthreads();
+        const double alpha = it.p[0], a = it.p[1], beta = it.p[2], c = it.p[3];
+        const double tl = bc[0], n_int = bc[3] - 3.0;
+        cd_k = alpha - a * bc[3] - a * c * n_int;
+        cd_lp = alpha * log(beta) - lgamma(alpha) - tl * beta - lgamma(a) - lgamma(c) +
+                lgamma(a + c) + bc[1] * (a - 1.0) + bc[2] * (a * c - 1.0) + cd_k * log(tl);
+        cd_k /= tl;
+    }
     for (int k = threadIdx.x; k < it.n; k += blockDim.x) {
         double gk = grad[k];
         if (it.prior == 1) {
             gk += -1.0 / it.prior_scale;
             lpr += -log(it.prior_scale) - xi[k] / it.prior_scale;
+        } else if (it.prior == 2) {
+            const double a = it.p[1], c = it.p[3];
+            gk += -it.p[2] + ((inl && inl[k]) ? a * c - 1.0 : a - 1.0) / xi[k] + cd_k;
         }
         gk *= scale_fac(xi[k], delta);
         gi[k] = gk;
         ri[k] += (eps * 0.5) * gk;
     }
     const double s = block_sum(lpr, red);
-    if (threadIdx.x == 0) lp[blockIdx.x] = out[it.out] + s;
+    if (threadIdx.x == 0) lp[blockIdx.x] = out[it.out] + s + cd_lp;
 }
 
 hipError_t launch_lf_pre(const HmcItem* items, int nitems, double* x, double* r, const double* g,
@@ -757,9 +793,9 @@ hipError_t launch_lf_pre(const HmcItem* items, int nitems, double* x, double* r,
 
 hipError_t launch_lf_post(const HmcItem* items, int nitems, const double* x, double* r, double* g,
                           const double* eps_delta, const double* out, double* lp,
-                          hipStream_t st) {
+                          const uint8_t* internal, hipStream_t st) {
     hipLaunchKernelGGL(k_lf_post, dim3(nitems), dim3(256), 0, st, items, x, r, g, eps_delta, out,
-                       lp);
+                       lp, internal);
     return hipGetLastError();
 }
 

@@ -36,7 +36,7 @@ namespace mcp {
 hipError_t launch_lf_pre(const HmcItem*, int, double*, double*, const double*, const double*,
                          double*, int, hipStream_t);
 hipError_t launch_lf_post(const HmcItem*, int, const double*, double*, double*, const double*,
-                          const double*, double*, hipStream_t);
+                          const double*, double*, const uint8_t*, hipStream_t);
 int set_last_error(int code, const char* msg);
 }  // namespace mcp
 
@@ -642,7 +642,7 @@ int enqueue_leapfrog(mcp_mlp* m) {
     MLP_TRY(launch_lf_pre(m->ditems, n, dx, dr, dg, ded, m->dX, 64 * std::max(1, d), st));
     infer(m, n);
     MLP_TRY(hipGetLastError());
-    MLP_TRY(launch_lf_post(m->ditems, n, dx, dr, dg, ded, m->dOut, m->dlp, st));
+    MLP_TRY(launch_lf_post(m->ditems, n, dx, dr, dg, ded, m->dOut, m->dlp, nullptr, st));
     MLP_TRY(hipMemcpyAsync(m->hstate, dx, 3 * T * sizeof(double), hipMemcpyDeviceToHost, st));
     MLP_TRY(hipMemcpyAsync(m->hstate + 3 * T + 2 * (size_t)n, m->dlp, n * sizeof(double),
                            hipMemcpyDeviceToHost, st));

@@ -111,6 +111,13 @@ class _Handle:
         """Exponential(mean=scale) prior on every branch of `item` for the leapfrog."""
         check(_lib.load_library().mcp_hmc_set_prior(self._h, int(item), 1, float(scale)))
 
+    def set_prior_compound_dirichlet(self, item: int, alpha: float, a: float, beta: float,
+                                     c: float) -> None:
+        """CompoundDirichlet(alpha, a, beta, c) tree-length prior of `item` for
+        the leapfrog (src/Likelihood/Prior.jl:1-37)."""
+        p = np.array([alpha, a, beta, c], dtype=np.float64)
+        check(_lib.load_library().mcp_hmc_set_prior_params(self._h, int(item), 2, dptr(p), 4))
+
     def leapfrog(self, eps_delta, x, r, g):
         """One device leapfrog for every item (see mcp_hmc_leapfrog).  x, r, g
         are updated in place (float64, blv_concat layout); returns

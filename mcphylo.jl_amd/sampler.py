@@ -273,15 +273,22 @@ class HostLeapfrog:
     """Leapfrog provider from a host function logfgrad(x) -> (lf, grad)
     (the CPU path; also used by the tests with toy targets)."""
 
-    def __init__(self, logfgrad, prior_scale: float | None = None):
+    def __init__(self, logfgrad, prior_scale: float | None = None, prior=None):
+        """prior_scale: exponential branch-length prior; prior: any callable
+        x -> (logprior, gradient) instead."""
         self.logfgrad = logfgrad
         self.prior_scale = prior_scale
+        self.prior = prior
 
     def one(self, x, r, g, eps, delta):
         from_oracle = _host_refraction(x, r, g, eps, delta)
         xm, r1 = from_oracle
         lf, gr = self.logfgrad(xm)
-        if self.prior_scale is not None:
+        if self.prior is not None:
+            lpr, gpr = self.prior(xm)
+            lf += lpr
+            gr = gr + gpr
+        elif self.prior_scale is not None:
             lf += float(np.sum(-math.log(self.prior_scale) - xm / self.prior_scale))
             gr = gr - 1.0 / self.prior_scale
         gr = gr * np.where(xm < delta, xm / delta, 1.0)

@@ -101,3 +101,32 @@ def logl_grad(ta, x, U, D, Uinv, mu, rates, pi, blv, weights=None, grad=True):
     if st != 0:
         raise MemoryError("oracle could not allocate its arrays")
     return (float(ll[0]), g) if grad else float(ll[0])
+
+
+def compound_dirichlet(blv, internal, alpha, a, beta, c):
+    """(logprior, gradient) of CompoundDirichlet(alpha, a, beta, c) at branch
+    lengths blv (internal_logpdf, src/Likelihood/Prior.jl:1-37; its gradient
+    in closed form).  internal[i] == 1 marks the branch above an internal
+    node."""
+    import math
+    blv = np.asarray(blv, dtype=np.float64)
+    inl = np.asarray(internal) == 1
+    T = 0.0
+    s_leaf = s_int = 0.0
+    nterm = 0.0
+    for i in range(blv.shape[0]):
+        T += blv[i]
+        if inl[i]:
+            s_int += math.log(blv[i])
+        else:
+            s_leaf += math.log(blv[i])
+            nterm += 1.0
+    n_int = nterm - 3.0
+    first = alpha * math.log(beta) - math.lgamma(alpha) - T * beta
+    second = -math.lgamma(a) - math.lgamma(c) + math.lgamma(a + c)
+    third = s_leaf * (a - 1.0) + s_int * (a * c - 1.0)
+    k = alpha - a * nterm - a * c * n_int
+    fourth = k * math.log(T)
+    g = np.array([-beta + ((a * c - 1.0) if inl[i] else (a - 1.0)) / blv[i] + k / T
+                  for i in range(blv.shape[0])])
+    return first + second + third + fourth, g

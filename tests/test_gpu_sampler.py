@@ -94,3 +94,30 @@ def test_nuts_on_simudata_runs_and_batches_exactly():
         c1 = S.NUTSChain(blv, np.random.default_rng(s), tree_depth=5)
         solo, *_ = S.run_nuts([c1], lf, 15, n_adapt=5)
         assert np.array_equal(solo[0], both[k])
+
+
+@pytest.mark.parametrize("eps", [0.0, 0.01, 0.2])
+def test_device_leapfrog_compound_dirichlet_prior_matches_host(eps):
+    """Second half of the device leapfrog with the CompoundDirichlet prior
+    (block sums of the tree length and of leaf / internal log lengths)
+    against the host restatement with the oracle's prior."""
+    t, x, ta, model, p = _simu_plan()
+    args = (2.0, 1.5, 0.4, 0.7)
+    p.set_prior_compound_dirichlet(0, *args)
+    inl = mcp.priors.internal_external(t)
+
+    def prior(b):
+        return oracle.compound_dirichlet(b, inl, *args)
+
+    rng = np.random.default_rng(9)
+    blv = mcp.get_branchlength_vector(t)
+    r = rng.standard_normal(blv.shape[0])
+    g = rng.standard_normal(blv.shape[0])
+    host = S.HostLeapfrog(_oracle_logfgrad(ta, x, model), prior=prior)
+    ref = host.one(blv.copy(), r.copy(), g.copy(), eps, 0.02)
+    xd, rd, gd = blv.copy(), r.copy(), g.copy()
+    lp, _ = p.leapfrog(np.array([eps, 0.02]), xd, rd, gd)
+    assert rel_err(xd, ref.x) < 1e-13
+    assert np.max(np.abs(gd - ref.g)) < 1e-8 * np.max(np.abs(ref.g))
+    assert np.max(np.abs(rd - ref.r)) < 1e-8 * max(1.0, np.max(np.abs(ref.r)))
+    assert rel_err(lp[0], ref.lf) < 1e-10
