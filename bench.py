@@ -66,12 +66,17 @@ def bench_batched(data, steps, warmup, sample_trees, schedule):
     plain single-tree launch sequence on a sample of the same trees."""
     datasets, items, models, blv = data
     b = mcp.Batch(datasets, items)
-    b.set_schedule(schedule)
+    b.set_schedule("levels" if schedule == "auto" else schedule)
     for i, m in enumerate(models):
         b.set_model(i, mcp.Model(*m))
     b.logl_grad(blv)
     tiles, tune_rows = b.autotune(reps=2)
     log(f"  autotuned tiles (columns per thread by state count): {tiles}")
+    sched_rows = []
+    if schedule == "auto":
+        _, sched_rows = b.autotune_schedule(reps=2)
+    sched, hwidth = b.schedule()
+    log(f"  schedule: {sched}" + (f" (hybrid width {hwidth})" if sched == "hybrid" else ""))
     n_items, total_patterns = b.sizes()
     for _ in range(warmup):
         b.logl_grad(blv)
@@ -82,25 +87,26 @@ def bench_batched(data, steps, warmup, sample_trees, schedule):
     b.set_timing(True)
     b.logl_grad(blv)
     ms, ph = b.launch_times()
+    lb = b.launch_bytes()
     b.set_timing(False)
     bytes_fwd, bytes_adj = b.algorithmic_bytes()
     b.close()
     fwd_ms, adj_ms = float(np.sum(ms[ph == 2])), float(np.sum(ms[ph == 3]))
     out = {"items": n_items, "patterns_per_step": total_patterns,
-           "schedule": schedule,
+           "schedule": sched, "hybrid_width": hwidth if sched == "hybrid" else None,
            "autotuned_cols_per_thread": {str(k): v for k, v in tiles.items()},
            "autotune_ms": [list(r) for r in tune_rows],
+           "autotune_schedule_ms": [list(r) for r in sched_rows],
            "ms_per_step": 1e3 * wall / steps,
            "value": total_patterns * steps / wall,
            "algorithmic_GB_per_step": (bytes_fwd + bytes_adj) / 1e9,
            "launches": int(ms.shape[0])}
-    if schedule == "fused":
-        out["fused_kernel_ms"] = adj_ms
-        out["fused_kernel_GBs"] = (bytes_fwd + bytes_adj) / (adj_ms * 1e-3) / 1e9
-    else:
-        out.update({"post_order_ms": fwd_ms, "pre_order_ms": adj_ms,
-                    "post_order_GBs": bytes_fwd / (fwd_ms * 1e-3) / 1e9,
-                    "pre_order_GBs": bytes_adj / (adj_ms * 1e-3) / 1e9})
+    if fwd_ms > 0:
+        out["post_order_ms"] = fwd_ms
+        out["post_order_GBs"] = float(np.sum(lb[ph == 2])) / (fwd_ms * 1e-3) / 1e9
+    if adj_ms > 0:
+        out["pre_order_ms"] = adj_ms
+        out["pre_order_GBs"] = float(np.sum(lb[ph == 3])) / (adj_ms * 1e-3) / 1e9
     # plain single-tree sequence (one launch per tree level) on the first
     # chain of the first trees
     pat, tsum = 0, 0.0
@@ -402,8 +408,8 @@ def main():
                     help="post-adaptation NUTS iterations of the ESS/s section (0 skips it)")
     ap.add_argument("--only-batched", action="store_true",
                     help="run only the ragged-batch section (for profiling it alone)")
-    ap.add_argument("--schedule", default="levels", choices=["levels", "fused"],
-                    help="launch schedule of the timed evaluations")
+    ap.add_argument("--schedule", default="auto", choices=["auto", "levels", "hybrid", "fused"],
+                    help="launch schedule of the timed evaluations (auto: autotuned)")
     ap.add_argument("--da-iters", type=int, default=100,
                     help="iterations of the delayed-acceptance section (200-taxon GTR); 0 skips")
     ap.add_argument("--dense-sites", type=int, default=100_000,
@@ -443,13 +449,18 @@ def main():
     blv = mcp.get_branchlength_vector(w.tree)
     plan = mcp.Plan(w.ta.parent, w.x, len(w.rates), weights=w.weights)
     plan.set_model(mcp.Model(w.U, w.D, w.Uinv, w.mu, w.rates, w.pi))
-    plan.set_schedule(args.schedule)
+    plan.set_schedule("levels" if args.schedule == "auto" else args.schedule)
     plan.logl_grad(blv)
     tiles, tune_rows = plan.autotune(reps=5)
     log(f"autotuned tiles (columns per thread by state count): {tiles}")
+    sched_rows = []
+    if args.schedule == "auto":
+        _, sched_rows = plan.autotune_schedule(reps=9)
+    sched, hwidth = plan.schedule()
+    log(f"schedule: {sched}" + (f" (hybrid width {hwidth})" if sched == "hybrid" else ""))
     for _ in range(args.warmup):
         plan.logl_grad(blv)
-    log(f"warmup done ({args.warmup}); timing {args.steps} steps ({args.schedule} schedule)")
+    log(f"warmup done ({args.warmup}); timing {args.steps} steps")
 
     # timed region: eager launches, no instrumentation
     t0 = time.perf_counter()
@@ -460,7 +471,7 @@ def main():
     log(f"eager: {1e3 * wall / args.steps:.3f} ms/step")
 
     # the same steps again with every kernel bracketed by HIP events on the
-    # engine's stream: per-kernel durations for the roofline
+    # engine's stream: per-launch durations and algorithmic bytes
     plan.set_timing(True)
     launch_ms = None
     t0 = time.perf_counter()
@@ -468,18 +479,22 @@ def main():
         plan.logl_grad(blv)
         ms, ph = plan.launch_times()
         launch_ms = ms if launch_ms is None else launch_ms + ms
+    lbytes = plan.launch_bytes()
     iwall = time.perf_counter() - t0
     plan.set_timing(False)
     bytes_fwd, bytes_adj = plan.algorithmic_bytes()
-    per_phase = {k: float(np.sum(launch_ms[ph == k])) / args.steps for k in (1, 2, 3, 4)}
+    launch_ms = launch_ms / args.steps
+    per_phase = {k: float(np.sum(launch_ms[ph == k])) for k in (1, 2, 3, 4)}
     n_phase = {k: int(np.sum(ph == k)) for k in (1, 2, 3, 4)}
-    if args.schedule == "fused":
-        kname = f"k_fused<{w.S},*,true>"
-        kbytes, kms, nl = bytes_fwd + bytes_adj, per_phase[3], n_phase[3]
-    elif per_phase[3] >= per_phase[2]:
-        kname, kbytes, kms, nl = f"k_level<{w.S},*,true> (pre-order)", bytes_adj, per_phase[3], n_phase[3]
-    else:
-        kname, kbytes, kms, nl = f"k_level<{w.S},*,false> (post-order)", bytes_fwd, per_phase[2], n_phase[2]
+    dom = 3 if per_phase[3] >= per_phase[2] else 2
+    kbytes, kms, nl = float(np.sum(lbytes[ph == dom])), per_phase[dom], n_phase[dom]
+    cpt = tiles.get(w.S, 0)
+    pass_name = "pre-order" if dom == 3 else "post-order"
+    kname = f"{pass_name} pass: k_level<{w.S},{cpt},{'true' if dom == 3 else 'false'}>"
+    if sched == "hybrid":
+        kname += " per wide level + one k_fused launch over the narrow top levels"
+    elif sched == "fused":
+        kname = f"k_fused<{w.S},{cpt},true> (both passes, one launch)"
     achieved = kbytes / (kms * 1e-3) / 1e9
 
     # graph replay of the same evaluation
@@ -494,17 +509,15 @@ def main():
     gvalue = w.C * args.steps / gwall
     log(f"graph: {1e3 * gwall / args.steps:.3f} ms/step")
 
-    # the other schedule, eager
-    other = "fused" if args.schedule == "levels" else "levels"
-    plan.set_schedule(other)
+    # the plain per-level schedule, eager, for reference
+    plan.set_schedule("levels")
     for _ in range(3):
         plan.logl_grad(blv)
     t2 = time.perf_counter()
     for _ in range(args.steps):
         plan.logl_grad(blv)
     owall = time.perf_counter() - t2
-    plan.set_schedule(args.schedule)
-    log(f"{other} schedule: {1e3 * owall / args.steps:.3f} ms/step")
+    log(f"levels schedule: {1e3 * owall / args.steps:.3f} ms/step")
 
     result = {
         "metric": "logL+grad site-pattern evals/sec",
@@ -529,17 +542,19 @@ def main():
                      "algorithmic_bytes_per_step": kbytes,
                      "kernel_ms_per_step": kms,
                      "avg_launch_us": 1e3 * kms / max(nl, 1)},
-        "schedule": args.schedule,
+        "schedule": sched,
+        "hybrid_width": hwidth if sched == "hybrid" else None,
         "autotuned_cols_per_thread": {str(k): v for k, v in tiles.items()},
         "autotune_ms": [list(r) for r in tune_rows],
+        "autotune_schedule_ms": [list(r) for r in sched_rows],
         "graph_value": gvalue,
         "instrumented_ms_per_step": 1e3 * iwall / args.steps,
         "kernel_ms_per_step": {"P_build": per_phase[1], "post_order": per_phase[2],
                                "pre_order": per_phase[3], "reduce": per_phase[4]},
         "launches_per_step": n_phase,
         "algorithmic_bytes_per_step": {"post_order": bytes_fwd, "pre_order": bytes_adj},
-        "other_schedule": {"schedule": other, "value": w.C * args.steps / owall,
-                           "ms_per_step": 1e3 * owall / args.steps},
+        "levels_schedule": {"value": w.C * args.steps / owall,
+                            "ms_per_step": 1e3 * owall / args.steps},
     }
     if batch_data is not None:
         log("timing ragged batch")

@@ -186,7 +186,18 @@ int mcp_use_graph(mcp_batch* batch, int32_t on);
  *                       depth), over every item (default). */
 #define MCP_SCHEDULE_FUSED 0
 #define MCP_SCHEDULE_LEVELS 1
+/*  MCP_SCHEDULE_HYBRID  per-level launches for the wide levels, one fused
+ *                       launch for every item's narrow top levels (post-order
+ *                       levels above, and pre-order levels from the root
+ *                       down to, the first level wider than the hybrid width);
+ *                       the narrow levels are consecutive in time, between
+ *                       the two passes. */
+#define MCP_SCHEDULE_HYBRID 2
 int mcp_set_schedule(mcp_batch* batch, int32_t mode);
+
+/* Widest level (internal nodes of one item) run fused by the HYBRID schedule
+ * (default 4). */
+int mcp_set_hybrid_width(mcp_batch* batch, int32_t width);
 
 /* Tile width of the level kernels for state count S: a task is one
  * wavefront over 64 * cpt consecutive patterns (cpt = columns per lane).
@@ -207,6 +218,14 @@ int mcp_get_tiles(mcp_batch* batch, int32_t S, int32_t* cpt);
  * The model of every item must be set. */
 int mcp_autotune(mcp_batch* batch, int32_t reps, double* report, int32_t cap, int32_t* n_report);
 
+/* Autotuner for the launch schedule: times LEVELS, HYBRID with widths 1, 2,
+ * 4, 8, 16 and FUSED the same way and keeps the fastest; report rows are
+ * (mode, hybrid width, ms).  mcp_get_schedule reads the choice (results are
+ * bitwise identical across schedules). */
+int mcp_autotune_schedule(mcp_batch* batch, int32_t reps, double* report, int32_t cap,
+                          int32_t* n_report);
+int mcp_get_schedule(mcp_batch* batch, int32_t* mode, int32_t* hybrid_width);
+
 /* Time later evaluations with HIP events on the engine's stream (on != 0).
  * ms[0] whole sequence (first upload to last download); ms[1..4] summed
  * kernel durations of the P/dP build, post-order levels, pre-order levels
@@ -220,6 +239,11 @@ int mcp_get_timing(mcp_batch* batch, double* ms5);
  * last timed evaluation, in launch order; returns the number of launches
  * (or a negative status).  At most cap entries are written. */
 int mcp_get_launch_times(mcp_batch* batch, double* ms, int32_t* phase, int32_t cap);
+
+/* Algorithmic HBM bytes of every launch of the last timed evaluation (same
+ * order as mcp_get_launch_times; 0 outside the level passes); returns the
+ * number of launches. */
+int mcp_get_launch_bytes(mcp_batch* batch, double* bytes, int32_t cap);
 
 /* Algorithmic HBM bytes of one evaluation under this engine's layout:
  * bytes[0] post-order levels, bytes[1] pre-order levels (compulsory slab

@@ -29,6 +29,7 @@ MCP_ERR_STATE = 7
 MCP_FLAG_COMPRESS_PATTERNS = 1
 MCP_SCHEDULE_FUSED = 0
 MCP_SCHEDULE_LEVELS = 1
+MCP_SCHEDULE_HYBRID = 2
 
 # Every symbol the header declares (checked by the CPU test suite).
 EXPORTED = (
@@ -36,7 +37,8 @@ EXPORTED = (
     "mcp_logl_grad", "mcp_batch_create", "mcp_batch_destroy", "mcp_batch_set_model",
     "mcp_batch_set_topology", "mcp_batch_logl", "mcp_batch_logl_grad", "mcp_use_graph",
     "mcp_set_schedule", "mcp_hmc_set_prior", "mcp_hmc_set_prior_params", "mcp_hmc_leapfrog",
-    "mcp_set_tiles", "mcp_get_tiles",
+    "mcp_set_tiles", "mcp_get_tiles", "mcp_set_hybrid_width", "mcp_autotune_schedule",
+    "mcp_get_schedule", "mcp_get_launch_bytes",
     "mcp_autotune",
     "mcp_set_timing", "mcp_get_timing", "mcp_get_launch_times", "mcp_get_bytes", "mcp_get_sizes",
     "mcp_mlp_create", "mcp_mlp_destroy", "mcp_mlp_layout", "mcp_mlp_get_params",
@@ -142,6 +144,10 @@ def load_library():
                                                      ctypes.c_int32]),
         "mcp_hmc_leapfrog": (ctypes.c_int, [vp, _dp, _dp, _dp, _dp, _dp, _dp]),
         "mcp_set_tiles": (ctypes.c_int, [vp, ctypes.c_int32, ctypes.c_int32]),
+        "mcp_set_hybrid_width": (ctypes.c_int, [vp, ctypes.c_int32]),
+        "mcp_autotune_schedule": (ctypes.c_int, [vp, ctypes.c_int32, _dp, ctypes.c_int32, _ip]),
+        "mcp_get_schedule": (ctypes.c_int, [vp, _ip, _ip]),
+        "mcp_get_launch_bytes": (ctypes.c_int, [vp, _dp, ctypes.c_int32]),
         "mcp_get_tiles": (ctypes.c_int, [vp, ctypes.c_int32, _ip]),
         "mcp_autotune": (ctypes.c_int, [vp, ctypes.c_int32, _dp, ctypes.c_int32, _ip]),
         "mcp_set_timing": (ctypes.c_int, [vp, ctypes.c_int32]),

@@ -192,6 +192,7 @@ struct ItemHost {
 struct Launch {
     bool adjoint;
     int S, cpt;
+    double bytes;       // algorithmic HBM bytes of the launch
     int job_begin, njobs;
     int taskoff_begin;  // index into task_off arena (njobs+1 entries)
     int ntasks;
@@ -205,6 +206,7 @@ struct mcp_engine {
     int device = 0;
     int waves = 4;     // wavefronts per block of the fused kernel
     int schedule = MCP_SCHEDULE_LEVELS;
+    int hybrid_width = 4;  // widest level (internal nodes) run fused under HYBRID
     int num_cu = 256;
     hipStream_t stream = nullptr;
     std::vector<DatasetHost> ds;
@@ -219,6 +221,7 @@ struct mcp_engine {
     std::vector<int32_t> taskoff, blkjob;
     struct FusedLaunch {
         int S, cpt, sched_begin, nitems, nblocks;
+        double bytes_f, bytes_a;  // algorithmic bytes of its post-/pre-order levels
     };
     // columns per thread chosen for each state count (-1: the built default);
     // set by mcp_set_tiles / mcp_autotune
@@ -249,6 +252,7 @@ struct mcp_engine {
     hipEvent_t ev_begin = nullptr, ev_end = nullptr;
     std::vector<hipEvent_t> kev;  // [2k] before launch k, [2k+1] after
     std::vector<int> kphase;      // phase of launch k: 1 P build, 2 post-order, 3 pre-order, 4 reduce
+    std::vector<double> kbytes;   // algorithmic bytes of launch k (0 outside the level passes)
     std::vector<double> kms;      // duration of launch k in the last timed evaluation
     int nk = 0;
     double last_ms[5] = {0, 0, 0, 0, 0};
@@ -692,22 +696,70 @@ static int rebuild_schedule(mcp_engine* e) {
         if (std::find(svals.begin(), svals.end(), it.S) == svals.end()) svals.push_back(it.S);
     }
     std::sort(svals.begin(), svals.end());
-    // node lists per (level) to avoid rescanning every item per level
+    // Per item, the levels run by the fused kernel: post-order heights >= h0
+    // and pre-order depths <= d0 (consecutive in time: the end of the
+    // post-order pass and the start of the pre-order one).  LEVELS: none;
+    // FUSED: all; HYBRID: the top levels of at most hybrid_width internal
+    // nodes, so narrow levels share one launch.
+    std::vector<int> h0(nitems), d0(nitems);
+    for (int i = 0; i < nitems; ++i) {
+        const ItemHost& it = e->items[i];
+        if (e->schedule == MCP_SCHEDULE_LEVELS) {
+            h0[i] = it.max_height + 1;
+            d0[i] = -1;
+        } else if (e->schedule == MCP_SCHEDULE_FUSED) {
+            h0[i] = 1;
+            d0[i] = it.max_depth;
+        } else {
+            std::vector<int> wh(it.max_height + 1, 0), wd(it.max_depth + 1, 0);
+            for (int v = 0; v < it.N; ++v)
+                if (it.nodes[v].nchild > 0) {
+                    wh[it.height[v]]++;
+                    wd[it.depth[v]]++;
+                }
+            int h = it.max_height;
+            while (h >= 1 && wh[h] <= e->hybrid_width) --h;
+            h0[i] = h + 1;
+            int dd = 0;
+            while (dd <= it.max_depth && wd[dd] <= e->hybrid_width) ++dd;
+            d0[i] = dd - 1;
+        }
+    }
+    // node lists per (level) of the per-level launches
     std::vector<std::vector<std::pair<int, int>>> by_h(maxh + 1), by_d(maxd + 1);
     for (int i = 0; i < nitems; ++i) {
         const ItemHost& it = e->items[i];
         for (int v = 0; v < it.N; ++v) {
             if (it.nodes[v].nchild == 0) continue;
-            by_h[it.height[v]].push_back({i, v});
-            by_d[it.depth[v]].push_back({i, v});
+            if (it.height[v] < h0[i]) by_h[it.height[v]].push_back({i, v});
+            if (it.depth[v] > d0[i]) by_d[it.depth[v]].push_back({i, v});
         }
     }
+    // algorithmic bytes of one job (the accounting of mcp_get_bytes, by job):
+    // post-order, node v reads its internal children's slabs (1 byte per
+    // pattern for a leaf child) and writes its own unless it is the root;
+    // pre-order, parent v reads its own pre-order slab unless it is the root,
+    // and per child reads the partial and writes the pre-order slab (a leaf
+    // child: its code)
+    auto job_bytes = [&](bool adjoint, int i, int v) -> double {
+        const ItemHost& it = e->items[i];
+        const DatasetHost& d = e->ds[it.dataset];
+        const double w = 8.0 * it.S * it.R * d.C;
+        const NodeInfo& n = it.nodes[v];
+        double b = (!n.is_root) ? w : 0.0;
+        for (int q = it.child_ptr[v]; q < it.child_ptr[v + 1]; ++q) {
+            const bool inner = it.nodes[it.children[q]].nchild > 0;
+            b += adjoint ? (inner ? 2.0 * w : (double)d.C) : (inner ? w : (double)d.C);
+        }
+        return b;
+    };
     auto add_launch = [&](bool adjoint, int S, const std::vector<std::pair<int, int>>& nodes) -> int {
         std::vector<LevelJob>& js = adjoint ? ja : jf;
         Launch L;
         L.adjoint = adjoint;
         L.S = S;
         L.cpt = e->cpt(S);
+        L.bytes = 0.0;
         L.job_begin = (int)js.size();
         L.taskoff_begin = (int)e->taskoff.size();
         long long tasks = 0;
@@ -716,6 +768,7 @@ static int rebuild_schedule(mcp_engine* e) {
             const ItemHost& it = e->items[i];
             if (it.S != S) continue;
             js.push_back(make_job(i, v));
+            L.bytes += job_bytes(adjoint, i, v);
             tasks += (long long)it.R * it.ntiles;
             e->taskoff.push_back((int32_t)tasks);
         }
@@ -742,18 +795,18 @@ static int rebuild_schedule(mcp_engine* e) {
         (adjoint ? e->adj : e->fwd).push_back(L);
         return MCP_OK;
     };
-    if (e->schedule == MCP_SCHEDULE_LEVELS) {
-        for (int h = 1; h <= maxh; ++h)
-            for (int S : svals)
-                if ((st = add_launch(false, S, by_h[h]))) return st;
-        for (int dd = 0; dd <= maxd; ++dd)
-            for (int S : svals)
-                if ((st = add_launch(true, S, by_d[dd]))) return st;
-    } else {
+    for (int h = 1; h <= maxh; ++h)
+        for (int S : svals)
+            if ((st = add_launch(false, S, by_h[h]))) return st;
+    for (int dd = 0; dd <= maxd; ++dd)
+        for (int S : svals)
+            if ((st = add_launch(true, S, by_d[dd]))) return st;
+    if (e->schedule != MCP_SCHEDULE_LEVELS) {
         // fused: jobs grouped by item, then level; per-item level offsets
         std::vector<int32_t> lf, la;
         std::vector<ItemSched> sched;
         std::vector<std::vector<ItemSched>> per_s(svals.size());
+        std::vector<std::pair<double, double>> fbytes(svals.size(), {0.0, 0.0});
         for (int i = 0; i < nitems; ++i) {
             const ItemHost& it = e->items[i];
             const DatasetHost& d = e->ds[it.dataset];
@@ -767,18 +820,25 @@ static int rebuild_schedule(mcp_engine* e) {
                 hv[it.height[v]].push_back(v);
                 dv[it.depth[v]].push_back(v);
             }
+            const size_t ks = std::find(svals.begin(), svals.end(), it.S) - svals.begin();
             I.lf0 = (int)lf.size();
-            for (int h = 1; h <= it.max_height; ++h) {
+            for (int h = h0[i]; h <= it.max_height; ++h) {
                 if (hv[h].empty()) continue;
                 lf.push_back((int32_t)jf.size());
-                for (int v : hv[h]) jf.push_back(make_job(i, v));
+                for (int v : hv[h]) {
+                    jf.push_back(make_job(i, v));
+                    fbytes[ks].first += job_bytes(false, i, v);
+                }
             }
             I.lf1 = (int)lf.size();
             I.la0 = (int)la.size();
-            for (int dd = 0; dd <= it.max_depth; ++dd) {
+            for (int dd = 0; dd <= d0[i]; ++dd) {
                 if (dv[dd].empty()) continue;
                 la.push_back((int32_t)ja.size());
-                for (int v : dv[dd]) ja.push_back(make_job(i, v));
+                for (int v : dv[dd]) {
+                    ja.push_back(make_job(i, v));
+                    fbytes[ks].second += job_bytes(true, i, v);
+                }
             }
             I.la1 = (int)la.size();
             const size_t k = std::find(svals.begin(), svals.end(), it.S) - svals.begin();
@@ -790,6 +850,8 @@ static int rebuild_schedule(mcp_engine* e) {
             mcp_engine::FusedLaunch F{};
             F.S = svals[k];
             F.cpt = e->cpt(svals[k]);
+            F.bytes_f = fbytes[k].first;
+            F.bytes_a = fbytes[k].second;
             F.sched_begin = (int)sched.size();
             long long blocks = 0;
             for (ItemSched I : per_s[k]) {
@@ -1025,6 +1087,7 @@ static int enqueue_eval(mcp_engine* e, bool grad) {
     hipStream_t st = e->stream;
     const bool tm = e->timing && !e->use_graph;
     e->nk = 0;
+    double launch_bytes = 0.0;  // algorithmic bytes of the next timed launch
     auto before = [&](int phase) -> int {
         if (!tm) return MCP_OK;
         const size_t k = (size_t)e->nk;
@@ -1034,7 +1097,10 @@ static int enqueue_eval(mcp_engine* e, bool grad) {
             e->kev.push_back(ev);
         }
         if (e->kphase.size() < k + 1) e->kphase.resize(k + 1);
+        if (e->kbytes.size() < k + 1) e->kbytes.resize(k + 1);
         e->kphase[k] = phase;
+        e->kbytes[k] = launch_bytes;
+        launch_bytes = 0.0;
         HIP_TRY(hipEventRecord(e->kev[2 * k], st));
         return MCP_OK;
     };
@@ -1057,25 +1123,30 @@ static int enqueue_eval(mcp_engine* e, bool grad) {
     TIMED(1, launch_tip_tables(e->dmodels.as<ModelDev>(), e->delemT.as<int32_t>(),
                                (int)e->items.size(), e->nelemT, st));
     for (const Launch& L : e->fwd) {
+        launch_bytes = L.bytes;
         TIMED(2, launch_level(false, L.S, L.cpt, e->djobs_f.as<LevelJob>() + L.job_begin,
                               e->dtaskoff.as<int32_t>() + L.taskoff_begin,
                               e->dblkjob.as<int32_t>() + L.blk_begin, L.ntasks, L.chunk, L.grid,
                               e->dparts.as<double>(), st));
     }
-    if (grad)
-        for (const Launch& L : e->adj) {
-            TIMED(3, launch_level(true, L.S, L.cpt, e->djobs_a.as<LevelJob>() + L.job_begin,
-                                  e->dtaskoff.as<int32_t>() + L.taskoff_begin,
-                                  e->dblkjob.as<int32_t>() + L.blk_begin, L.ntasks, L.chunk,
-                                  L.grid, e->dparts.as<double>(), st));
-        }
+    // the fused levels sit between the two passes (top of the post-order
+    // pass, start of the pre-order pass)
     for (const auto& F : e->fused) {
+        launch_bytes = F.bytes_f + (grad ? F.bytes_a : 0.0);
         TIMED(grad ? 3 : 2,
               launch_fused(grad, F.S, F.cpt, e->dsched.as<ItemSched>() + F.sched_begin, F.nitems,
                            F.nblocks, e->waves, e->djobs_f.as<LevelJob>(), e->dlvl_f.as<int32_t>(),
                            e->djobs_a.as<LevelJob>(), e->dlvl_a.as<int32_t>(),
                            e->dparts.as<double>(), st));
     }
+    if (grad)
+        for (const Launch& L : e->adj) {
+            launch_bytes = L.bytes;
+            TIMED(3, launch_level(true, L.S, L.cpt, e->djobs_a.as<LevelJob>() + L.job_begin,
+                                  e->dtaskoff.as<int32_t>() + L.taskoff_begin,
+                                  e->dblkjob.as<int32_t>() + L.blk_begin, L.ntasks, L.chunk,
+                                  L.grid, e->dparts.as<double>(), st));
+        }
     const int nops = grad ? (int)e->red.size() : e->n_ll_ops;
     const int nops2 = grad ? (int)e->red2.size() : e->n_ll_ops2;
     TIMED(4, launch_reduce(e->dred.as<RedOp>(), nops, e->dparts.as<double>(),
@@ -1208,16 +1279,16 @@ static int enqueue_leapfrog(mcp_engine* e) {
                              e->dtaskoff.as<int32_t>() + L.taskoff_begin,
                              e->dblkjob.as<int32_t>() + L.blk_begin, L.ntasks, L.chunk, L.grid,
                              e->dparts.as<double>(), st));
-    for (const Launch& L : e->adj)
-        HIP_TRY(launch_level(true, L.S, L.cpt, e->djobs_a.as<LevelJob>() + L.job_begin,
-                             e->dtaskoff.as<int32_t>() + L.taskoff_begin,
-                             e->dblkjob.as<int32_t>() + L.blk_begin, L.ntasks, L.chunk, L.grid,
-                             e->dparts.as<double>(), st));
     for (const auto& F : e->fused)
         HIP_TRY(launch_fused(true, F.S, F.cpt, e->dsched.as<ItemSched>() + F.sched_begin, F.nitems,
                              F.nblocks, e->waves, e->djobs_f.as<LevelJob>(),
                              e->dlvl_f.as<int32_t>(), e->djobs_a.as<LevelJob>(),
                              e->dlvl_a.as<int32_t>(), e->dparts.as<double>(), st));
+    for (const Launch& L : e->adj)
+        HIP_TRY(launch_level(true, L.S, L.cpt, e->djobs_a.as<LevelJob>() + L.job_begin,
+                             e->dtaskoff.as<int32_t>() + L.taskoff_begin,
+                             e->dblkjob.as<int32_t>() + L.blk_begin, L.ntasks, L.chunk, L.grid,
+                             e->dparts.as<double>(), st));
     HIP_TRY(launch_reduce(e->dred.as<RedOp>(), (int)e->red.size(), e->dparts.as<double>(),
                           e->dout.as<double>(), e->dredtmp.as<double>(), st));
     HIP_TRY(launch_reduce(e->dred2.as<RedOp>(), (int)e->red2.size(), e->dredtmp.as<double>(),
@@ -1432,13 +1503,23 @@ int mcp_use_graph(mcp_batch* b, int32_t on) {
 
 int mcp_set_schedule(mcp_batch* b, int32_t mode) {
     if (!b) return fail(MCP_ERR_INVALID, "null handle");
-    if (mode != MCP_SCHEDULE_FUSED && mode != MCP_SCHEDULE_LEVELS)
+    if (mode != MCP_SCHEDULE_FUSED && mode != MCP_SCHEDULE_LEVELS && mode != MCP_SCHEDULE_HYBRID)
         return fail(MCP_ERR_INVALID, "unknown schedule");
     DeviceGuard guard(b->device);
     HIP_TRY(hipStreamSynchronize(b->stream));
     if (b->schedule == mode) return MCP_OK;
     b->schedule = mode;
     return rebuild_schedule(b);
+}
+
+int mcp_set_hybrid_width(mcp_batch* b, int32_t width) {
+    if (!b) return fail(MCP_ERR_INVALID, "null handle");
+    if (width < 1) return fail(MCP_ERR_INVALID, "hybrid width must be >= 1");
+    DeviceGuard guard(b->device);
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    if (b->hybrid_width == width) return MCP_OK;
+    b->hybrid_width = width;
+    return b->schedule == MCP_SCHEDULE_HYBRID ? rebuild_schedule(b) : MCP_OK;
 }
 
 int mcp_set_tiles(mcp_batch* b, int32_t S, int32_t cpt) {
@@ -1521,6 +1602,71 @@ int mcp_autotune(mcp_batch* b, int32_t reps, double* report, int32_t cap, int32_
     return st ? st : st2;
 }
 
+int mcp_autotune_schedule(mcp_batch* b, int32_t reps, double* report, int32_t cap,
+                          int32_t* n_report) {
+    if (!b) return fail(MCP_ERR_INVALID, "null handle");
+    if (reps < 1 || reps > 1000) return fail(MCP_ERR_INVALID, "reps must be in 1..1000");
+    for (auto& it : b->items)
+        if (!it.model_set) return fail(MCP_ERR_STATE, "substitution model not set");
+    DeviceGuard guard(b->device);
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    std::vector<double> blv((size_t)b->total_blv, 0.1);
+    if (b->evaluated) std::memcpy(blv.data(), b->hblv.p, blv.size() * sizeof(double));
+    std::vector<double> ll(b->items.size()), grad((size_t)b->total_blv);
+    const bool graph = b->use_graph, timing = b->timing;
+    b->use_graph = false;
+    b->timing = false;
+    // candidates: (schedule, hybrid width)
+    const int cands[][2] = {{MCP_SCHEDULE_LEVELS, 0}, {MCP_SCHEDULE_HYBRID, 1},
+                            {MCP_SCHEDULE_HYBRID, 2}, {MCP_SCHEDULE_HYBRID, 4},
+                            {MCP_SCHEDULE_HYBRID, 8}, {MCP_SCHEDULE_HYBRID, 16},
+                            {MCP_SCHEDULE_FUSED, 0}};
+    const int keep_w = b->hybrid_width;
+    int best_mode = b->schedule, best_w = keep_w, nrep = 0, st = MCP_OK;
+    double best_ms = 1e300;
+    for (const auto& c : cands) {
+        b->schedule = c[0];
+        b->hybrid_width = c[0] == MCP_SCHEDULE_HYBRID ? c[1] : keep_w;
+        if ((st = rebuild_schedule(b))) break;
+        if ((st = engine_eval(b, blv.data(), ll.data(), grad.data(), true))) break;
+        std::vector<double> t(reps);
+        for (int k = 0; k < reps && !st; ++k) {
+            const auto t0 = std::chrono::steady_clock::now();
+            st = engine_eval(b, blv.data(), ll.data(), grad.data(), true);
+            t[k] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0)
+                       .count();
+        }
+        if (st) break;
+        std::sort(t.begin(), t.end());
+        const double ms = t[reps / 2];
+        if (report && nrep < cap) {
+            report[3 * nrep] = c[0];
+            report[3 * nrep + 1] = c[1];
+            report[3 * nrep + 2] = ms;
+        }
+        ++nrep;
+        if (ms < best_ms) {
+            best_ms = ms;
+            best_mode = c[0];
+            best_w = b->hybrid_width;
+        }
+    }
+    b->schedule = best_mode;
+    b->hybrid_width = best_w;
+    const int st2 = rebuild_schedule(b);
+    b->use_graph = graph;
+    b->timing = timing;
+    if (n_report) *n_report = nrep;
+    return st ? st : st2;
+}
+
+int mcp_get_schedule(mcp_batch* b, int32_t* mode, int32_t* hybrid_width) {
+    if (!b || !mode) return fail(MCP_ERR_INVALID, "null argument");
+    *mode = b->schedule;
+    if (hybrid_width) *hybrid_width = b->hybrid_width;
+    return MCP_OK;
+}
+
 int mcp_set_timing(mcp_batch* b, int32_t on) {
     if (!b) return fail(MCP_ERR_INVALID, "null handle");
     DeviceGuard guard(b->device);
@@ -1545,6 +1691,13 @@ int mcp_get_launch_times(mcp_batch* b, double* ms, int32_t* phase, int32_t cap) 
         if (ms) ms[k] = b->kms[k];
         if (phase) phase[k] = b->kphase[k];
     }
+    return (int)b->kms.size();
+}
+
+int mcp_get_launch_bytes(mcp_batch* b, double* bytes, int32_t cap) {
+    if (!b) return -fail(MCP_ERR_INVALID, "null handle");
+    const int n = std::min<int>(cap, (int)b->kms.size());
+    for (int k = 0; k < n && bytes; ++k) bytes[k] = k < (int)b->kbytes.size() ? b->kbytes[k] : 0.0;
     return (int)b->kms.size();
 }
 

@@ -60,9 +60,14 @@ class _Handle:
     def use_graph(self, on: bool = True) -> None:
         check(_lib.load_library().mcp_use_graph(self._h, 1 if on else 0))
 
-    def set_schedule(self, mode: str) -> None:
-        """'fused' (one launch per evaluation) or 'levels' (one launch per tree level)."""
-        m = {"fused": _lib.MCP_SCHEDULE_FUSED, "levels": _lib.MCP_SCHEDULE_LEVELS}[mode]
+    def set_schedule(self, mode: str, hybrid_width: int | None = None) -> None:
+        """'fused' (one launch per evaluation), 'levels' (one launch per tree
+        level) or 'hybrid' (per-level launches for wide levels, one fused
+        launch for the narrow top levels of at most hybrid_width nodes)."""
+        m = {"fused": _lib.MCP_SCHEDULE_FUSED, "levels": _lib.MCP_SCHEDULE_LEVELS,
+             "hybrid": _lib.MCP_SCHEDULE_HYBRID}[mode]
+        if hybrid_width is not None:
+            check(_lib.load_library().mcp_set_hybrid_width(self._h, int(hybrid_width)))
         check(_lib.load_library().mcp_set_schedule(self._h, m))
 
     def set_tiles(self, S: int, cpt: int) -> None:
@@ -74,6 +79,36 @@ class _Handle:
         out = np.zeros(1, dtype=np.int32)
         check(_lib.load_library().mcp_get_tiles(self._h, int(S), iptr(out)))
         return int(out[0])
+
+    def schedule(self):
+        """(mode name, hybrid width) of the current launch schedule."""
+        m = np.zeros(1, dtype=np.int32)
+        wv = np.zeros(1, dtype=np.int32)
+        check(_lib.load_library().mcp_get_schedule(self._h, iptr(m), iptr(wv)))
+        return {0: "fused", 1: "levels", 2: "hybrid"}[int(m[0])], int(wv[0])
+
+    def autotune_schedule(self, reps: int = 5):
+        """Time levels / hybrid (widths 1..16) / fused and keep the fastest.
+        Returns ((mode, width), [(mode, width, ms), ...])."""
+        cap = 16
+        rep = np.zeros(3 * cap)
+        n = np.zeros(1, dtype=np.int32)
+        check(_lib.load_library().mcp_autotune_schedule(self._h, int(reps), dptr(rep), cap,
+                                                        iptr(n)))
+        names = {0: "fused", 1: "levels", 2: "hybrid"}
+        rows = [(names[int(rep[3 * k])], int(rep[3 * k + 1]), float(rep[3 * k + 2]))
+                for k in range(min(int(n[0]), cap))]
+        return self.schedule(), rows
+
+    def launch_bytes(self) -> np.ndarray:
+        """Algorithmic bytes of every launch of the last timed evaluation."""
+        lib = _lib.load_library()
+        n = lib.mcp_get_launch_bytes(self._h, None, 0)
+        if n < 0:
+            check(-n)
+        out = np.zeros(n)
+        lib.mcp_get_launch_bytes(self._h, dptr(out), n)
+        return out
 
     def autotune(self, reps: int = 5):
         """Time every built tile width per state count on this engine's data

@@ -73,3 +73,49 @@ def test_autotune_mixed_batch_keeps_results_and_reports_candidates():
     assert np.array_equal(ll3, ll1) and np.array_equal(g3, g1)
     b.close()
     b2.close()
+
+
+@pytest.mark.parametrize("width", [1, 2, 4, 8])
+def test_hybrid_schedule_is_bitwise_levels(width):
+    data = mcp.synthetic.make_batched_workload(n_trees=5, chains=2, n_range=(16, 80),
+                                               c_range=(300, 3000))
+    ds, items, models, blv = data
+    res = {}
+    for mode in ("levels", "hybrid", "fused"):
+        b = mcp.Batch(ds, items)
+        for i, m in enumerate(models):
+            b.set_model(i, mcp.Model(*m))
+        b.set_schedule(mode, hybrid_width=width)
+        assert b.schedule()[0] == mode
+        ll, g = b.logl_grad(blv)
+        l_only = b.logl(blv)
+        b.set_timing(True)
+        b.logl_grad(blv)
+        ms, ph = b.launch_times()
+        by = b.launch_bytes()
+        b.set_timing(False)
+        fwd, adj = b.algorithmic_bytes()
+        assert by.shape == ms.shape
+        assert abs(by.sum() - (fwd + adj)) <= 1e-9 * (fwd + adj)
+        res[mode] = (ll, g, l_only)
+        b.close()
+    for mode in ("hybrid", "fused"):
+        assert np.array_equal(res[mode][0], res["levels"][0]), mode
+        assert np.array_equal(res[mode][1], res["levels"][1]), mode
+        assert np.array_equal(res[mode][2], res["levels"][2]), mode
+
+
+def test_hybrid_schedule_single_tree_vs_oracle_and_schedule_autotune():
+    w, blv, llo, go = _case(505, 64, 4, 900, 4)
+    p = mcp.Plan(w.ta.parent, w.x, 4)
+    p.set_model(mcp.Model(w.U, w.D, w.Uinv, w.mu, w.rates, w.pi))
+    p.set_schedule("hybrid", hybrid_width=3)
+    ll, g = p.logl_grad(blv)
+    assert rel_err(ll, llo) < LL_TOL and grad_err(g, go) < GRAD_TOL
+    (mode, width), rows = p.autotune_schedule(reps=2)
+    assert len(rows) == 7
+    best = min(rows, key=lambda r: r[2])
+    assert mode == best[0] and (mode != "hybrid" or width == best[1])
+    ll2, g2 = p.logl_grad(blv)
+    assert ll2 == ll and np.array_equal(g2, g)
+    p.close()
