@@ -324,6 +324,43 @@ int mcp_mlp_leapfrog(mcp_mlp* mlp, int32_t chains, double prior_scale, const dou
                      double* x, double* r, double* g, double* logpost);
 int mcp_mlp_use_graph(mcp_mlp* mlp, int32_t on);
 
+/* ---- native NUTS driver (many chains in lockstep) ------------------------
+ * Restates the tree NUTS of src/samplers/pnuts.jl:103-419 (nuts_sub!,
+ * buildtree, multinomial sampling, dual averaging) and nutsepsilon
+ * (src/samplers/tree_hamiltonian/auxilliary.jl:6-51) for a fixed topology, in
+ * C++: one batched leapfrog per round over all chains, so chains of a batch
+ * share every launch and no host interpreter sits between the rounds.
+ * Chain i owns dims[i] branch lengths at offset sum(dims[0..i)) of the
+ * concatenated vectors.  The leapfrog provider is an engine (items =
+ * chains; mcp_hmc_leapfrog), a surrogate (mcp_mlp_leapfrog) or a callback
+ * with the same contract: for every chain advance (x, r, g) by one leapfrog
+ * of (eps, delta) = eps_delta[2i..2i+1] and write the log posterior. */
+typedef struct mcp_nuts mcp_nuts;
+typedef int (*mcp_leapfrog_fn)(void* ctx, int32_t n_chains, const double* eps_delta, double* x,
+                               double* r, double* g, double* logpost);
+
+/* seeds: [n_chains] generator seeds; target: acceptance target of the step
+ * size adaptation; molifier: 1 as the reference (delta = 2 eps), 0 for
+ * pure reflection at zero. */
+int mcp_nuts_create(mcp_nuts** out, int32_t n_chains, const int32_t* dims, const double* x0,
+                    const uint64_t* seeds, double target, int32_t tree_depth, int32_t molifier);
+void mcp_nuts_destroy(mcp_nuts* nuts);
+int mcp_nuts_use_batch(mcp_nuts* nuts, mcp_batch* batch);
+int mcp_nuts_use_mlp(mcp_nuts* nuts, mcp_mlp* mlp, double prior_scale);
+int mcp_nuts_use_callback(mcp_nuts* nuts, mcp_leapfrog_fn fn, void* ctx);
+
+/* n_iter transitions of every chain (adapt != 0: with step-size adaptation).
+ * draws: [n_iter][sum dims] or NULL; logpost: [n_iter][n_chains] or NULL;
+ * stats (may be NULL): {leapfrog rounds, leapfrogs summed over chains}. */
+int mcp_nuts_run(mcp_nuts* nuts, int32_t n_iter, int32_t adapt, double* draws, double* logpost,
+                 int64_t* stats);
+
+/* Current positions, step sizes, log posteriors and last tree depths (any
+ * may be NULL); mcp_nuts_set overwrites positions / step sizes / log
+ * posteriors (NULL: keep). */
+int mcp_nuts_get(mcp_nuts* nuts, double* x, double* eps, double* lf, int32_t* depth);
+int mcp_nuts_set(mcp_nuts* nuts, const double* x, const double* eps, const double* lf);
+
 /* Message of the last failed call on this host thread ("" if none). */
 const char* mcp_last_error(void);
 

@@ -36,7 +36,8 @@ import time
 import numpy as np
 
 __all__ = ["HState", "NUTSChain", "run_nuts", "ess_bm", "mcse_bm", "EngineLeapfrog",
-           "HostLeapfrog", "SurrogateLeapfrog", "run_da_nuts", "exponential_logprior"]
+           "HostLeapfrog", "SurrogateLeapfrog", "run_da_nuts", "exponential_logprior",
+           "NativeNUTS", "run_da_nuts_native"]
 
 
 class HState:
@@ -526,3 +527,146 @@ def ess_bm(x, size: int = 100) -> float:
     if se == 0.0:
         return float(x.shape[0])
     return float(min((sd / se) ** 2, x.shape[0]))
+
+
+class NativeNUTS:
+    """The native lockstep NUTS driver (mcp_nuts_*): the same sampler as
+    NUTSChain/run_nuts, with every chain's trajectory building in C++ and one
+    batched leapfrog per round.  Chains may have different numbers of
+    branches (the items of a ragged batch)."""
+
+    def __init__(self, x0, seeds, target: float = 0.8, tree_depth: int = 10,
+                 molifier: bool = True):
+        import ctypes
+        from . import _lib
+        self._lib = _lib
+        self.dims = np.array([len(x) for x in x0], dtype=np.int32)
+        self.off = np.concatenate([[0], np.cumsum(self.dims)]).astype(np.int64)
+        x = np.ascontiguousarray(np.concatenate([np.asarray(v, dtype=np.float64) for v in x0]))
+        sd = np.ascontiguousarray(seeds, dtype=np.uint64)
+        if sd.shape[0] != self.dims.shape[0]:
+            raise ValueError("one seed per chain")
+        self._h = ctypes.c_void_p()
+        _lib.check(_lib.load_library().mcp_nuts_create(
+            ctypes.byref(self._h), len(self.dims), _lib.iptr(self.dims), _lib.dptr(x),
+            sd.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), float(target), int(tree_depth),
+            1 if molifier else 0))
+        self._keep = None
+        self.n = len(self.dims)
+        self.total = int(self.off[-1])
+
+    def close(self):
+        if self._h and self._h.value:
+            self._lib.load_library().mcp_nuts_destroy(self._h)
+            import ctypes
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def use_engine(self, engine) -> None:
+        self._keep = engine
+        self._lib.check(self._lib.load_library().mcp_nuts_use_batch(self._h, engine.handle))
+
+    def use_surrogate(self, surrogate, prior_scale: float | None = None) -> None:
+        self._keep = surrogate
+        self._lib.check(self._lib.load_library().mcp_nuts_use_mlp(
+            self._h, surrogate._h, -1.0 if prior_scale is None else float(prior_scale)))
+
+    def use_callback(self, leapfrog) -> None:
+        """leapfrog(eps_delta [2n], x, r, g [total], logpost [n]) updates the
+        arrays in place (numpy views of the driver's buffers)."""
+        total, n = self.total, self.n
+
+        def cb(ctx, nch, ed, x, r, g, lp):
+            try:
+                leapfrog(np.ctypeslib.as_array(ed, (2 * n,)), np.ctypeslib.as_array(x, (total,)),
+                         np.ctypeslib.as_array(r, (total,)), np.ctypeslib.as_array(g, (total,)),
+                         np.ctypeslib.as_array(lp, (n,)))
+                return 0
+            except Exception:
+                return 1
+        self._keep = self._lib.LEAPFROG_FN(cb)
+        self._lib.check(self._lib.load_library().mcp_nuts_use_callback(self._h, self._keep, None))
+
+    def run(self, n_iter: int, adapt: bool = False, keep_draws: bool = True):
+        """n_iter transitions of every chain.  Returns (draws [chain] ->
+        [n_iter, dim], logpost [n_iter, n_chains], wall seconds, rounds,
+        leapfrogs)."""
+        import ctypes
+        d = np.zeros((n_iter, self.total)) if keep_draws else None
+        lp = np.zeros((n_iter, self.n))
+        st = np.zeros(2, dtype=np.int64)
+        t0 = time.perf_counter()
+        self._lib.check(self._lib.load_library().mcp_nuts_run(
+            self._h, int(n_iter), 1 if adapt else 0, self._lib.dptr(d), self._lib.dptr(lp),
+            st.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))))
+        wall = time.perf_counter() - t0
+        draws = None if d is None else [d[:, self.off[i]:self.off[i + 1]] for i in range(self.n)]
+        return draws, lp, wall, int(st[0]), int(st[1])
+
+    def state(self):
+        """(x concatenated, step sizes, log posteriors, last tree depths)."""
+        x = np.zeros(self.total)
+        eps, lf = np.zeros(self.n), np.zeros(self.n)
+        dep = np.zeros(self.n, dtype=np.int32)
+        self._lib.check(self._lib.load_library().mcp_nuts_get(
+            self._h, self._lib.dptr(x), self._lib.dptr(eps), self._lib.dptr(lf),
+            self._lib.iptr(dep)))
+        return x, eps, lf, dep
+
+    def set_state(self, x=None, eps=None, lf=None) -> None:
+        def arr(a):
+            return None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+        x, eps, lf = arr(x), arr(eps), arr(lf)
+        self._lib.check(self._lib.load_library().mcp_nuts_set(
+            self._h, self._lib.dptr(x), self._lib.dptr(eps), self._lib.dptr(lf)))
+
+
+def run_da_nuts_native(nuts: NativeNUTS, surrogate, prior_scale, exact_logpost, n_iter: int,
+                       n_adapt: int = 0, trace=None):
+    """Delayed-acceptance NUTS with the native driver as stage one (on the
+    surrogate posterior, `nuts` set up with use_surrogate) and one batched
+    exact evaluation per iteration as stage two; the acceptance is that of
+    run_da_nuts.  Returns (draws [chain] -> [n_iter, dim], exact lp
+    [n_iter, n_chains], wall seconds, stats)."""
+    n, d = nuts.n, int(nuts.dims[0])
+    t0 = time.perf_counter()
+    x, _, _, _ = nuts.state()
+    X = x.reshape(n, d)
+    ed = np.tile([0.0, 1e-300], n)
+    xx, rr, gg = X.copy(), np.zeros((n, d)), np.zeros((n, d))
+    sur = surrogate.leapfrog(ed, xx, rr, gg, prior_scale)
+    lp = np.asarray(exact_logpost(X), dtype=np.float64).copy()
+    draws = np.zeros((n_iter, n, d))
+    lps = np.zeros((n_iter, n))
+    stats = {"exact_evals": n, "proposals": 0, "accepted": 0, "rounds": 1}
+    rng = np.random.default_rng(int(nuts.dims.sum()) + 7)
+    for it in range(n_adapt + n_iter):
+        x_old = X.copy()
+        _, _, _, rounds, _ = nuts.run(1, adapt=it < n_adapt, keep_draws=False)
+        stats["rounds"] += rounds
+        xn, _, sur_new, _ = nuts.state()
+        Xn = xn.reshape(n, d)
+        lp_new = np.asarray(exact_logpost(Xn), dtype=np.float64)
+        stats["exact_evals"] += n
+        la = (lp_new - sur_new) - (lp - sur)
+        acc = (la >= 0.0) | (np.isfinite(la) & (rng.random(n) < np.exp(np.minimum(la, 0.0))))
+        X = np.where(acc[:, None], Xn, x_old)
+        lp = np.where(acc, lp_new, lp)
+        sur = np.where(acc, sur_new, sur)
+        if not acc.all():
+            nuts.set_state(x=X.reshape(-1), lf=sur)
+        stats["proposals"] += n
+        stats["accepted"] += int(acc.sum())
+        if it >= n_adapt:
+            draws[it - n_adapt] = X
+            lps[it - n_adapt] = lp
+        if trace is not None:
+            trace(it + 1)
+    wall = time.perf_counter() - t0
+    stats["accept_rate"] = stats["accepted"] / max(1, stats["proposals"])
+    return [draws[:, i, :] for i in range(n)], lps, wall, stats

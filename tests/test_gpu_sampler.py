@@ -121,3 +121,75 @@ def test_device_leapfrog_compound_dirichlet_prior_matches_host(eps):
     assert np.max(np.abs(gd - ref.g)) < 1e-8 * np.max(np.abs(ref.g))
     assert np.max(np.abs(rd - ref.r)) < 1e-8 * max(1.0, np.max(np.abs(ref.r)))
     assert rel_err(lp[0], ref.lf) < 1e-10
+
+
+def _simu_batch(nch, prior=0.1):
+    t, x = load_fixture("simudata")
+    ta = mcp.tree_arrays(t)
+    U, D, Ui, mu = mcp.JC(np.ones(4) / 4)
+    leaf = ta.leaf_nums
+    b = mcp.Batch([{"x_leaves": x[:, :, leaf - 1], "leaf_nums": leaf}],
+                  [{"dataset": 0, "parent": ta.parent, "n_rates": 1}] * nch)
+    for i in range(nch):
+        b.set_model(i, mcp.Model(U, D, Ui, mu, [1.0], np.ones(4) / 4))
+        b.set_prior_exponential(i, prior)
+    b.use_graph(True)
+    return t, b
+
+
+def test_native_nuts_on_engine_matches_host_driver_in_distribution():
+    nch = 4
+    t, b = _simu_batch(nch)
+    blv = mcp.get_branchlength_vector(t)
+    nuts = S.NativeNUTS([blv] * nch, [31, 32, 33, 34], target=0.8, tree_depth=6,
+                        molifier=False)
+    nuts.use_engine(b)
+    nuts.run(50, adapt=True, keep_draws=False)
+    dn, lpn, wall_n, rounds_n, _ = nuts.run(400)
+    chains = [S.NUTSChain(blv, np.random.default_rng(40 + i), target=0.8, tree_depth=6,
+                          molifier=False) for i in range(nch)]
+    dh, _, wall_h, rounds_h = S.run_nuts(chains, S.EngineLeapfrog(b, nch, [17] * nch), 400,
+                                         n_adapt=50)
+    A, B = np.concatenate(dn), np.concatenate(dh)
+    for k in range(17):
+        se = np.hypot(S.mcse_bm(A[:, k]), S.mcse_bm(B[:, k]))
+        assert abs(A[:, k].mean() - B[:, k].mean()) < 4.5 * se, k
+    assert np.all(np.isfinite(lpn))
+    print(f"native {wall_n / rounds_n * 1e3:.3f} ms/round, host {wall_h / rounds_h * 1e3:.3f}",
+          flush=True)
+
+
+def test_native_delayed_acceptance_on_simudata():
+    nch, d, prior = 4, 17, 0.1
+    t, b = _simu_batch(nch, prior)
+    blv = mcp.get_branchlength_vector(t)
+    rec = []
+    lf = S.EngineLeapfrog(b, nch, [d] * nch, record=rec)
+    warm = [S.NUTSChain(blv, np.random.default_rng(100 + i), target=0.8, tree_depth=6,
+                        molifier=False) for i in range(nch)]
+    S.run_nuts(warm, lf, 100, n_adapt=100)
+    X = np.array([r[1] for r in rec])
+    y = np.array([r[2] for r in rec])
+    s = mcp.Surrogate(d, 256, 0, 1)
+    s.fit_scaling(X, y)
+    s.set_data(X, y)
+    s.train(60, batch=256, lr=2e-3, lr_final=1e-5, seed=2)
+    s.use_graph(True)
+
+    def exact_logpost(Xs):
+        return b.logl(Xs.reshape(-1)) + S.exponential_logprior(Xs, prior)
+    nuts = S.NativeNUTS([c.x for c in warm], [51, 52, 53, 54], target=0.8, tree_depth=6,
+                        molifier=False)
+    nuts.set_state(eps=np.array([c.epsilon for c in warm]))
+    nuts.use_surrogate(s, prior)
+    draws, lps, wall, st = S.run_da_nuts_native(nuts, s, prior, exact_logpost, 800)
+    assert st["accept_rate"] > 0.6, st
+    nuts2 = S.NativeNUTS([c.x for c in warm], [61, 62, 63, 64], target=0.8, tree_depth=6,
+                         molifier=False)
+    nuts2.set_state(eps=np.array([c.epsilon for c in warm]))
+    nuts2.use_engine(b)
+    dp, *_ = nuts2.run(800)
+    A, B = np.concatenate(draws), np.concatenate(dp)
+    for k in range(d):
+        se = np.hypot(S.mcse_bm(A[:, k]), S.mcse_bm(B[:, k]))
+        assert abs(A[:, k].mean() - B[:, k].mean()) < 4.5 * se, k
