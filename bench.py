@@ -134,8 +134,9 @@ def bench_batched(data, steps, warmup, sample_trees, schedule):
 
 
 def nuts_run(chain, leapfrog, n_adapt, n_iter):
-    """Adaptation, then n_iter timed NUTS transitions; ESS (batch means,
-    batch size 100) of tree length and log posterior per sampling second."""
+    """Adaptation, then n_iter timed NUTS transitions with the host driver;
+    ESS (batch means, batch size 100) of tree length and log posterior per
+    sampling second."""
     S = mcp.sampler
     S.run_nuts([chain], leapfrog, 0, n_adapt=n_adapt)
     log(f"  adapted: step size {chain.epsilon:.3g}")
@@ -151,23 +152,53 @@ def nuts_run(chain, leapfrog, n_adapt, n_iter):
             "mean_tree_depth": float(np.mean(chain.depths[-n_iter:]))}
 
 
-def bench_nuts(n_adapt, n_iter, cpu: bool):
-    """Plain NUTS on the 20-taxon restriction-site workload: device leapfrog
-    (one HIP graph per step) and, for the baseline, the same sampler with the
-    CPU oracle's logL + gradient."""
+def native_run(nuts, n_adapt, n_iter):
+    """Adaptation, then n_iter timed transitions of every chain with the native
+    driver; per chain ESS (batch size 100) of tree length and log posterior,
+    summed over chains, per sampling second."""
+    S = mcp.sampler
+    nuts.run(n_adapt, adapt=True, keep_draws=False)
+    draws, lp, wall, rounds, lfs = nuts.run(n_iter)
+    ess = [min(S.ess_bm(d.sum(axis=1)), S.ess_bm(lp[:, i])) for i, d in enumerate(draws)]
+    tl = np.concatenate([d.sum(axis=1) for d in draws])
+    _, eps, _, _ = nuts.state()
+    return {"chains": nuts.n, "iterations": n_iter, "wall_s": wall, "leapfrog_rounds": rounds,
+            "leapfrogs": lfs, "ess_sum": float(np.sum(ess)), "ess_min_chain": float(np.min(ess)),
+            "ess_per_s": float(np.sum(ess)) / wall, "mean_tree_length": float(tl.mean()),
+            "mean_step_size": float(np.mean(eps)), "ms_per_round": 1e3 * wall / max(rounds, 1)}
+
+
+def bench_nuts(n_adapt, n_iter, cpu: bool, batch_chains: int = 64):
+    """NUTS on the 20-taxon restriction-site workload: the native driver with
+    the device leapfrog (one HIP graph per round) for one chain and for
+    `batch_chains` chains sharing every launch; the baseline is the same
+    sampler on one chain with the CPU oracle's logL + gradient."""
     S = mcp.sampler
     w = mcp.synthetic.make_workload("restriction20")
     blv = mcp.get_branchlength_vector(w.tree)
-    plan = mcp.Plan(w.ta.parent, w.x, 1)
-    plan.set_model(mcp.Model(w.U, w.D, w.Uinv, w.mu, w.rates, w.pi))
-    plan.set_prior_exponential(0, 0.1)
-    plan.use_graph(True)
+    leaf = w.ta.leaf_nums
     out = {"workload": "20-taxon Yule tree, restriction-site model pi=(.5,.5), 1000 simulated "
                        "sites, R=1, exponential(0.1) branch-length prior, NUTS target 0.8, "
-                       "tree depth 5 (the reference's PNUTS test setting)"}
-    c = S.NUTSChain(blv, np.random.default_rng(20261202), target=0.8, tree_depth=5)
-    out["gpu"] = nuts_run(c, S.EngineLeapfrog(plan, 1, [blv.shape[0]]), n_adapt, n_iter)
-    plan.close()
+                       "tree depth 5 (the reference's PNUTS test setting); ESS = min over tree "
+                       "length and log posterior per chain, summed over chains"}
+    # a single chain's ESS from the reference estimator (batches of 100
+    # draws) needs many batches: the single GPU chain runs 10x the iterations,
+    # the 64-chain batch and the CPU baseline 2x
+    for key, nch, iters in (("gpu", 1, 10 * n_iter), ("gpu_batched", batch_chains, 2 * n_iter)):
+        b = mcp.Batch([{"x_leaves": w.x[:, :, leaf - 1], "leaf_nums": leaf}],
+                      [{"dataset": 0, "parent": w.ta.parent, "n_rates": 1}] * nch)
+        for i in range(nch):
+            b.set_model(i, mcp.Model(w.U, w.D, w.Uinv, w.mu, w.rates, w.pi))
+            b.set_prior_exponential(i, 0.1)
+        b.use_graph(True)
+        nuts = S.NativeNUTS([blv] * nch, [20261202 + i for i in range(nch)], target=0.8,
+                            tree_depth=5)
+        nuts.use_engine(b)
+        out[key] = native_run(nuts, n_adapt, iters)
+        log(f"  {key}: {out[key]['ess_per_s']:.1f} ESS/s over {nch} chain(s)")
+        nuts.close()
+        b.close()
+    out["batched_over_single_ess_per_s"] = out["gpu_batched"]["ess_per_s"] / out["gpu"]["ess_per_s"]
     if cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle
@@ -176,9 +207,11 @@ def bench_nuts(n_adapt, n_iter, cpu: bool):
         def logfgrad(x):
             return oracle.logl_grad(w.ta, w.x, w.U, w.D, w.Uinv, w.mu, w.rates, w.pi, x)
         c2 = S.NUTSChain(blv, np.random.default_rng(20261202), target=0.8, tree_depth=5)
-        out["cpu"] = nuts_run(c2, S.HostLeapfrog(logfgrad, prior_scale=0.1), n_adapt, n_iter)
+        out["cpu"] = nuts_run(c2, S.HostLeapfrog(logfgrad, prior_scale=0.1), n_adapt, 2 * n_iter)
         out["cpu"]["threads"] = min(os.cpu_count() or 1, 16)
         out["gpu_over_cpu_ess_per_s"] = out["gpu"]["ess_per_s"] / out["cpu"]["ess_per_s"]
+        out["gpu_batched_over_cpu_ess_per_s"] = (out["gpu_batched"]["ess_per_s"]
+                                                 / out["cpu"]["ess_per_s"])
     return out
 
 
@@ -248,15 +281,21 @@ def bench_da(n_warm, n_iter, n_chains=8, prior=0.1, depth=8, grad_weight=1e4, ep
                 "min_ess_branch": ess_b, "ess_per_s": min(ess_tl, ess_lp) / wall,
                 "mean_tree_length": float(tl.mean())}
 
-    plain = [S.NUTSChain(c.x, np.random.default_rng(200 + i), target=0.8, tree_depth=depth,
-                         epsilon=c.epsilon, molifier=False) for i, c in enumerate(warm)]
-    dp, lpp, wall_p, rounds_p = S.run_nuts(plain, lf, n_iter, n_adapt=0)
+    x_warm = [c.x for c in warm]
+    eps_warm = np.array([c.epsilon for c in warm])
+    plain = S.NativeNUTS(x_warm, [200 + i for i in range(n_chains)], target=0.8,
+                         tree_depth=depth, molifier=False)
+    plain.set_state(eps=eps_warm)
+    plain.use_engine(b)
+    dp, lp_mat, wall_p, rounds_p, lfs_p = plain.run(n_iter)
+    lpp = [lp_mat[:, i] for i in range(n_chains)]
     out = {"workload": f"{len(ta.leaf_nums)} taxa, 4-state GTR, {w.C} weighted patterns, "
                        f"4 gamma cats, exponential({prior}) branch-length prior, {n_chains} chains "
-                       f"in lockstep, NUTS target 0.8, tree depth {depth}, molifier off",
+                       f"in lockstep (native driver), NUTS target 0.8, tree depth {depth}, "
+                       "molifier off",
            "warmup_iterations": n_warm, "iterations": n_iter, "training_pairs": int(len(L)),
            "plain": summary(dp, lpp, wall_p)}
-    out["plain"]["exact_leapfrogs"] = int(sum(c.leapfrogs for c in plain))
+    out["plain"]["exact_leapfrogs"] = int(lfs_p)
     log(f"  plain: {wall_p:.1f} s, ESS/s {out['plain']['ess_per_s']:.2f}")
     s = mcp.Surrogate(d, 256, 0, 1)
     s.fit_scaling(X, L)
@@ -267,13 +306,15 @@ def bench_da(n_warm, n_iter, n_chains=8, prior=0.1, depth=8, grad_weight=1e4, ep
     Dp = np.concatenate(dp)
     err = s.predict(Dp) - (np.concatenate(lpp) - S.exponential_logprior(Dp, prior))
     s.use_graph(True)
-    da = [S.NUTSChain(c.x, np.random.default_rng(300 + i), target=0.8, tree_depth=depth,
-                      epsilon=c.epsilon, molifier=False) for i, c in enumerate(warm)]
+    da = S.NativeNUTS(x_warm, [300 + i for i in range(n_chains)], target=0.8, tree_depth=depth,
+                      molifier=False)
+    da.set_state(eps=eps_warm)
+    da.use_surrogate(s, prior)
 
     def exact_logpost(Xs):
         return b.logl(Xs.reshape(-1)) + S.exponential_logprior(Xs, prior)
-    dd, lpd, wall_d, st = S.run_da_nuts(da, S.SurrogateLeapfrog(s, n_chains, prior),
-                                        exact_logpost, n_iter, n_adapt=0)
+    dd, lp_da, wall_d, st = S.run_da_nuts_native(da, s, prior, exact_logpost, n_iter)
+    lpd = [lp_da[:, i] for i in range(n_chains)]
     out["da"] = summary(dd, lpd, wall_d)
     out["da"].update({"second_stage_accept": st["accept_rate"],
                       "exact_evaluations": int(st["exact_evals"]),
