@@ -139,6 +139,17 @@ int mcp_batch_logl(mcp_batch* batch, const double* blv_concat, double* ll);
 int mcp_batch_logl_grad(mcp_batch* batch, const double* blv_concat, double* ll,
                         double* grad_concat);
 
+/* Per-(branch, category) components of the gradient: grad_vr [N-1][R]
+ * (category fastest; per item, concatenated for a batch), so that
+ * grad[v] = sum_r grad_vr[v][r].  Because P of branch v in category r
+ * depends on rho_r t_v only, d logL / d rho_r = sum_v t_v grad_vr[v][r] / rho_r
+ * -- the gradient with respect to the rate parameters (and, through
+ * discrete_gamma_rates, Rates.jl:11-37, the gamma shape).  No reference
+ * counterpart: FelsensteinFunction/6 returns branch gradients only. */
+int mcp_logl_grad_by_rate(mcp_plan* plan, const double* blv, double* ll, double* grad_vr);
+int mcp_batch_logl_grad_by_rate(mcp_batch* batch, const double* blv_concat, double* ll,
+                                double* grad_vr_concat);
+
 /* ---- device leapfrog (NUTS / HMC over branch lengths) ------------------- */
 
 /* Branch-length prior added to the log posterior of item `item` by the

@@ -235,6 +235,13 @@ struct mcp_engine {
     // the chunks (ll ops first)
     std::vector<RedOp> red, red2;
     int n_ll_ops = 0, n_ll_ops2 = 0;
+    // per-(branch, category) gradient components: one sum over the tiles of
+    // each (item, branch, category) into dout_vr (layout [item][branch][r])
+    std::vector<RedOp> red_vr;
+    std::vector<size_t> vr_off;
+    size_t total_vr = 0;
+    DevBuf dred_vr, dout_vr;
+    PinnedBuf hout_vr;
     long long nelemP = 0, nelemT = 0;
     size_t total_blv = 0, total_out = 0;
     long long total_patterns = 0;
@@ -905,6 +912,23 @@ static int rebuild_schedule(mcp_engine* e) {
         if ((st = upload(e, e->dred2, e->red2.data(), e->red2.size() * sizeof(RedOp)))) return st;
         if ((st = e->dredtmp.ensure((size_t)ntmp * sizeof(double)))) return st;
     }
+    e->red_vr.clear();
+    e->vr_off.assign(e->items.size(), 0);
+    e->total_vr = 0;
+    for (size_t i = 0; i < e->items.size(); ++i) {
+        const ItemHost& it = e->items[i];
+        e->vr_off[i] = e->total_vr;
+        for (int v = 0; v < it.N - 1; ++v)
+            for (int r = 0; r < it.R; ++r)
+                e->red_vr.push_back(RedOp{(int32_t)(e->total_vr + (size_t)v * it.R + r),
+                                          it.parts_off + (it.nint + v) * it.R * it.ntiles +
+                                              r * it.ntiles,
+                                          it.ntiles, 1});
+        e->total_vr += (size_t)(it.N - 1) * it.R;
+    }
+    if ((st = upload(e, e->dred_vr, e->red_vr.data(), e->red_vr.size() * sizeof(RedOp)))) return st;
+    if ((st = e->dout_vr.ensure(std::max<size_t>(e->total_vr, 1) * sizeof(double)))) return st;
+    if ((st = e->hout_vr.ensure(std::max<size_t>(e->total_vr, 1) * sizeof(double)))) return st;
     if ((st = upload(e, e->dred, e->red.data(), e->red.size() * sizeof(RedOp)))) return st;
     return MCP_OK;
 }
@@ -1083,7 +1107,7 @@ static int engine_set_topology(mcp_engine* e, int item, const int32_t* parent) {
 // Enqueue one evaluation on the engine's stream (no synchronisation).  In
 // timing mode (eager launches only) every kernel launch is bracketed by a
 // pair of HIP events on the same stream.
-static int enqueue_eval(mcp_engine* e, bool grad) {
+static int enqueue_eval(mcp_engine* e, bool grad, bool by_rate = false) {
     hipStream_t st = e->stream;
     const bool tm = e->timing && !e->use_graph;
     e->nk = 0;
@@ -1147,13 +1171,20 @@ static int enqueue_eval(mcp_engine* e, bool grad) {
                                   e->dblkjob.as<int32_t>() + L.blk_begin, L.ntasks, L.chunk,
                                   L.grid, e->dparts.as<double>(), st));
         }
-    const int nops = grad ? (int)e->red.size() : e->n_ll_ops;
-    const int nops2 = grad ? (int)e->red2.size() : e->n_ll_ops2;
+    const int nops = (grad && !by_rate) ? (int)e->red.size() : e->n_ll_ops;
+    const int nops2 = (grad && !by_rate) ? (int)e->red2.size() : e->n_ll_ops2;
     TIMED(4, launch_reduce(e->dred.as<RedOp>(), nops, e->dparts.as<double>(),
                            e->dout.as<double>(), e->dredtmp.as<double>(), st));
     if (nops2 > 0) {
         TIMED(4, launch_reduce(e->dred2.as<RedOp>(), nops2, e->dredtmp.as<double>(),
                                e->dout.as<double>(), e->dredtmp.as<double>(), st));
+    }
+    if (grad && by_rate) {
+        TIMED(4, launch_reduce(e->dred_vr.as<RedOp>(), (int)e->red_vr.size(),
+                               e->dparts.as<double>(), e->dout.as<double>(),
+                               e->dout_vr.as<double>(), st));
+        HIP_TRY(hipMemcpyAsync(e->hout_vr.p, e->dout_vr.p, e->total_vr * sizeof(double),
+                               hipMemcpyDeviceToHost, st));
     }
 #undef TIMED
     HIP_TRY(hipMemcpyAsync(e->hout.p, e->dout.p, e->total_out * sizeof(double),
@@ -1216,6 +1247,27 @@ static int engine_eval(mcp_engine* e, const double* blv, double* ll, double* gra
         if (want_grad && grad) std::memcpy(grad + goff, o + it.out_off + 1, (it.N - 1) * sizeof(double));
         goff += it.N - 1;
     }
+    return MCP_OK;
+}
+
+// logL and the per-(branch, category) components of the gradient (eager
+// launches): grad_vr[item][v][r], summing over r gives the branch gradient.
+static int engine_eval_by_rate(mcp_engine* e, const double* blv, double* ll, double* grad_vr) {
+    if (!e || !blv || !ll || !grad_vr) return fail(MCP_ERR_INVALID, "null argument");
+    for (auto& it : e->items)
+        if (!it.model_set) return fail(MCP_ERR_STATE, "substitution model not set");
+    DeviceGuard guard(e->device);
+    std::memcpy(e->hblv.p, blv, e->total_blv * sizeof(double));
+    e->evaluated = true;
+    const bool tm = e->timing;
+    e->timing = false;
+    int st = enqueue_eval(e, true, true);
+    e->timing = tm;
+    if (st) return st;
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    const double* o = e->hout.as<double>();
+    for (size_t i = 0; i < e->items.size(); ++i) ll[i] = o[e->items[i].out_off];
+    std::memcpy(grad_vr, e->hout_vr.p, e->total_vr * sizeof(double));
     return MCP_OK;
 }
 
@@ -1461,6 +1513,15 @@ int mcp_hmc_set_prior(mcp_batch* b, int32_t item, int32_t kind, double scale) {
     HIP_TRY(hipMemcpy(b->dhmc_items.as<HmcItem>() + item, &b->hmc_items[item], sizeof(HmcItem),
                       hipMemcpyHostToDevice));
     return MCP_OK;
+}
+
+int mcp_logl_grad_by_rate(mcp_plan* p, const double* blv, double* ll, double* grad_vr) {
+    return engine_eval_by_rate(p, blv, ll, grad_vr);
+}
+
+int mcp_batch_logl_grad_by_rate(mcp_batch* b, const double* blv_concat, double* ll,
+                                double* grad_vr_concat) {
+    return engine_eval_by_rate(b, blv_concat, ll, grad_vr_concat);
 }
 
 int mcp_hmc_set_prior_params(mcp_batch* b, int32_t item, int32_t kind, const double* params,

@@ -242,6 +242,24 @@ class Plan(_Handle):
         check(_lib.load_library().mcp_logl_grad(self._h, dptr(b), dptr(out), dptr(g)))
         return float(out[0]), g
 
+    def logl_grad_by_rate(self, blv):
+        """(ll, G [N-1, R]): per-(branch, category) gradient components;
+        G.sum(axis=1) is the branch gradient."""
+        b = np.ascontiguousarray(blv, dtype=np.float64)
+        if b.shape != (self.N - 1,):
+            raise _lib.McpDimensionMismatch(_lib.MCP_ERR_DIMENSION, "blv must have length N-1")
+        out = np.zeros(1)
+        g = np.zeros((self.N - 1, self.R))
+        check(_lib.load_library().mcp_logl_grad_by_rate(self._h, dptr(b), dptr(out), dptr(g)))
+        return float(out[0]), g
+
+    def logl_grad_rates(self, blv, rates):
+        """(ll, d ll / d rates [R]): gradient with respect to the category
+        rates, sum_v t_v G[v, r] / rho_r."""
+        ll, G = self.logl_grad_by_rate(blv)
+        b = np.asarray(blv, dtype=np.float64)
+        return ll, (b @ G) / np.asarray(rates, dtype=np.float64)
+
 
 class Batch(_Handle):
     """Ragged batch of items.
@@ -335,6 +353,19 @@ class Batch(_Handle):
         g = np.zeros(b.shape[0])
         check(_lib.load_library().mcp_batch_logl_grad(self._h, dptr(b), dptr(ll), dptr(g)))
         return ll, g
+
+    def logl_grad_by_rate(self, blv_concat):
+        """(ll [items], [G_i [N_i-1, R_i]]) per-(branch, category) gradient
+        components of every item."""
+        b = self._blv(blv_concat)
+        ll = np.zeros(self.n_items)
+        sizes = [(n - 1) * r for n, r in zip(self.item_nodes, self.item_R)]
+        g = np.zeros(int(np.sum(sizes)))
+        check(_lib.load_library().mcp_batch_logl_grad_by_rate(self._h, dptr(b), dptr(ll),
+                                                              dptr(g)))
+        off = np.concatenate([[0], np.cumsum(sizes)])
+        return ll, [g[off[i]:off[i + 1]].reshape(self.item_nodes[i] - 1, self.item_R[i])
+                    for i in range(self.n_items)]
 
     def split(self, grad_concat):
         return [grad_concat[self.blv_off[i]:self.blv_off[i + 1]] for i in range(self.n_items)]
