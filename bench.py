@@ -339,6 +339,58 @@ def bench_da(n_warm, n_iter, n_chains=8, prior=0.1, depth=8, grad_weight=1e4, ep
     return out
 
 
+def train_flops(rows, d, H, grad_matching):
+    """Algorithmic matrix-product flops of `rows` training rows of the
+    d -> H -> H -> 1 network: forward, parameter backward and, with gradient
+    matching, the second backward pass through the input gradient."""
+    if grad_matching:
+        return 2.0 * rows * H * (5 * d + 6 * H)
+    return 2.0 * rows * H * (2 * d + 3 * H)
+
+
+def bench_train(d=398, H=256, n=65536, epochs=4, batches=(256, 1024), grad_weight=1e4):
+    """Surrogate training throughput (BF16 MFMA, FP32 accumulation) on a
+    synthetic smooth target; the kernels' work does not depend on the values."""
+    rng = np.random.default_rng(20261202)
+    X = np.exp(rng.normal(-2.5, 0.5, (n, d)))
+    A = rng.normal(0, 1.0 / np.sqrt(d), (d, 8))
+    L = np.log(X)
+    u = (L + 2.5) @ A
+    y = -5000.0 - 0.5 * np.sum(u * u, axis=1)
+    G = -(u @ A.T) / X
+    out = {"workload": f"{d} inputs -> {H} -> {H} -> 1 tanh MLP + separable quadratic skip, "
+                       f"{n} resident training rows, value + gradient matching "
+                       f"(weight {grad_weight:g}), Adam",
+           "flops_per_row": train_flops(1, d, H, True), "runs": []}
+    for batch in batches:
+        s = mcp.Surrogate(d, H, 0, 1)
+        s.fit_scaling(X, y)
+        s.set_data(X, y, G)
+        s.train(1, batch=batch, lr=1e-3, lr_final=1e-3, seed=1, grad_weight=grad_weight)
+        t0 = time.perf_counter()
+        losses = s.train(epochs, batch=batch, lr=1e-3, lr_final=1e-4, seed=2,
+                         grad_weight=grad_weight)
+        dt = time.perf_counter() - t0
+        s.train_graph(False)
+        t1 = time.perf_counter()
+        s.train(1, batch=batch, lr=1e-4, lr_final=1e-4, seed=3, grad_weight=grad_weight)
+        dt_eager = time.perf_counter() - t1
+        steps = epochs * (-(-n // batch))
+        tf = train_flops(n, d, H, True) * epochs / dt / 1e12
+        out["runs"].append({"batch": batch, "epochs": epochs, "wall_s": dt,
+                            "ms_per_step": 1e3 * dt / steps, "rows_per_s": n * epochs / dt,
+                            "tflops": tf, "eager_ms_per_step": 1e3 * dt_eager / (steps / epochs),
+                            "final_loss": float(losses[-1])})
+        log(f"  train batch {batch}: {1e3 * dt / steps:.3f} ms/step, {tf:.2f} TFLOP/s "
+            f"(eager {1e3 * dt_eager / (steps / epochs):.3f} ms/step)")
+        s.close()
+    best = max(out["runs"], key=lambda r: r["tflops"])
+    out["roofline"] = {"bound": "mfma", "achieved": best["tflops"], "peak": 2500.0,
+                       "unit": "TFLOP/s", "frac": best["tflops"] / 2500.0, "traffic": None,
+                       "batch": best["batch"]}
+    return out
+
+
 def bench_dense(n_sites, steps, cpu: bool, cpu_seconds: float):
     """The dense 20-state workload (1000 taxa, n_sites sites as leaf state
     codes, 4 gamma categories) on the FP64-MFMA level kernels; the CPU oracle
@@ -451,6 +503,11 @@ def main():
                     help="run only the ragged-batch section (for profiling it alone)")
     ap.add_argument("--schedule", default="auto", choices=["auto", "levels", "hybrid", "fused"],
                     help="launch schedule of the timed evaluations (auto: autotuned)")
+    ap.add_argument("--hybrid-width", type=int, default=4,
+                    help="widest level fused into the top launch with --schedule hybrid")
+    ap.add_argument("--cols-per-thread", type=int, default=0,
+                    help="fixed tile width of the level kernels (0: autotuned); with a fixed "
+                         "--schedule this pins the configuration a profiler sees")
     ap.add_argument("--da-iters", type=int, default=100,
                     help="iterations of the delayed-acceptance section (200-taxon GTR); 0 skips")
     ap.add_argument("--dense-sites", type=int, default=100_000,
@@ -458,6 +515,10 @@ def main():
     ap.add_argument("--dense-steps", type=int, default=5)
     ap.add_argument("--only-dense", action="store_true",
                     help="run only the dense 20-state section (for profiling it alone)")
+    ap.add_argument("--train-epochs", type=int, default=4,
+                    help="epochs of the surrogate-training section (0 skips it)")
+    ap.add_argument("--only-train", action="store_true",
+                    help="run only the surrogate-training section (for profiling it alone)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -478,6 +539,10 @@ def main():
                           min(args.cpu_seconds, 8.0))
         print(json.dumps({"dense20": res}), flush=True)
         return
+    if args.only_train:
+        res = bench_train(epochs=max(1, args.train_epochs))
+        print(json.dumps({"surrogate_training": res}), flush=True)
+        return
     if args.only_batched:
         if batch_data is None:
             log("--only-batched needs --batch-trees > 0")
@@ -490,10 +555,17 @@ def main():
     blv = mcp.get_branchlength_vector(w.tree)
     plan = mcp.Plan(w.ta.parent, w.x, len(w.rates), weights=w.weights)
     plan.set_model(mcp.Model(w.U, w.D, w.Uinv, w.mu, w.rates, w.pi))
-    plan.set_schedule("levels" if args.schedule == "auto" else args.schedule)
+    plan.set_schedule("levels" if args.schedule == "auto" else args.schedule,
+                      args.hybrid_width if args.schedule == "hybrid" else None)
     plan.logl_grad(blv)
-    tiles, tune_rows = plan.autotune(reps=5)
-    log(f"autotuned tiles (columns per thread by state count): {tiles}")
+    if args.cols_per_thread > 0:
+        n_states = int(np.asarray(w.U).shape[0])
+        plan.set_tiles(n_states, args.cols_per_thread)
+        tiles, tune_rows = {n_states: plan.tiles(n_states)}, []
+        log(f"fixed tiles (columns per thread by state count): {tiles}")
+    else:
+        tiles, tune_rows = plan.autotune(reps=5)
+        log(f"autotuned tiles (columns per thread by state count): {tiles}")
     sched_rows = []
     if args.schedule == "auto":
         _, sched_rows = plan.autotune_schedule(reps=9)
@@ -585,7 +657,8 @@ def main():
                      "avg_launch_us": 1e3 * kms / max(nl, 1)},
         "schedule": sched,
         "hybrid_width": hwidth if sched == "hybrid" else None,
-        "autotuned_cols_per_thread": {str(k): v for k, v in tiles.items()},
+        "cols_per_thread": {str(k): v for k, v in tiles.items()},
+        "tiles_autotuned": args.cols_per_thread <= 0,
         "autotune_ms": [list(r) for r in tune_rows],
         "autotune_schedule_ms": [list(r) for r in sched_rows],
         "graph_value": gvalue,
@@ -612,6 +685,9 @@ def main():
         log("delayed-acceptance section")
         result["delayed_acceptance"] = bench_da(100, args.da_iters)
         log(f"DA: {result['delayed_acceptance']['da_over_plain_ess_per_s']:.2f}x plain ESS/s")
+    if args.train_epochs > 0:
+        log("surrogate-training section")
+        result["surrogate_training"] = bench_train(epochs=args.train_epochs)
     if args.dense_sites > 0:
         log("dense 20-state section")
         result["dense20"] = bench_dense(args.dense_sites, args.dense_steps, not args.no_cpu,

@@ -335,6 +335,12 @@ int mcp_mlp_leapfrog(mcp_mlp* mlp, int32_t chains, double prior_scale, const dou
                      double* x, double* r, double* g, double* logpost);
 int mcp_mlp_use_graph(mcp_mlp* mlp, int32_t on);
 
+/* Training runs each full minibatch as one replay of a captured HIP graph
+ * (default on; the minibatch offset and Adam's rate come from a device step
+ * counter).  Off launches the same kernels one by one: same results, bit for
+ * bit. */
+int mcp_mlp_train_graph(mcp_mlp* mlp, int32_t on);
+
 /* ---- native NUTS driver (many chains in lockstep) ------------------------
  * Restates the tree NUTS of src/samplers/pnuts.jl:103-419 (nuts_sub!,
  * buildtree, multinomial sampling, dual averaging) and nutsepsilon

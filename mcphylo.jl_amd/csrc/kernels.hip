@@ -372,10 +372,13 @@ __device__ __forceinline__ void adj_task(const LevelJob& J, int r, int tile,
 #ifndef MCP_ADJ_WAVES_PER_EU
 #define MCP_ADJ_WAVES_PER_EU 1
 #endif
+#ifndef MCP_FWD_WAVES_PER_EU
+#define MCP_FWD_WAVES_PER_EU 1
+#endif
 
 template <int S, int CPT, bool ADJ>
 __global__ void __launch_bounds__(64)
-__attribute__((amdgpu_waves_per_eu(ADJ ? MCP_ADJ_WAVES_PER_EU : 1))) k_level(const LevelJob* __restrict__ jobs,
+__attribute__((amdgpu_waves_per_eu(ADJ ? MCP_ADJ_WAVES_PER_EU : MCP_FWD_WAVES_PER_EU))) k_level(const LevelJob* __restrict__ jobs,
                                               const int32_t* __restrict__ task_off,
                                               const int32_t* __restrict__ blk_job, int total,
                                               int chunk, double* __restrict__ parts) {

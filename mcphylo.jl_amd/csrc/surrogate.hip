@@ -47,65 +47,152 @@ namespace {
 typedef float fvec4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
-constexpr int kTile = 64;   // block tile (M and N)
-constexpr int kKStep = 32;  // K per MFMA step
-constexpr int kLdsK = 40;   // LDS row length in bf16 (32 + 8 padding)
+constexpr int kTile = 64;    // block tile (M and N)
+constexpr int kKStep = 32;   // K per MFMA step
+constexpr int kKChunk = 128; // K staged per barrier pair
+constexpr int kLdsK = kKChunk + 8;  // LDS row length in bf16 (16-byte aligned rows)
 
-enum { EPI_STORE = 0, EPI_BIAS_TANH = 1, EPI_MUL_DTANH = 2, EPI_ACCUM = 3 };
+// GEMM epilogues (o = m * ldc + n):
+//   EPI_STORE      C = acc
+//   EPI_BIAS_TANH  C = tanh(acc + aux[n])
+//   EPI_MUL_DTANH  C = acc (1 - aux[o]^2)
+//   EPI_SOB_T1     C = acc;  out2 = acc (1 - aux[o]^2)                 (T1, delta1)
+//   EPI_SOB_K1     C = acc (1 - h^2);  out2 = -2 h acc aux2[o], h = aux[o]   (mu1, Gam1)
+//   EPI_SOB_K2     h = aux[o], s2 = 1 - h^2, g = vm[m]:
+//                  out2 = g aux2[o] - 2 h acc vn[n] s2;  C = g h + acc s2   (A2tot, w3 rows)
+//   EPI_SOB_A1     C = (acc + aux2[o]) (1 - aux[o]^2)                   (A1tot)
+enum {
+    EPI_STORE = 0,
+    EPI_BIAS_TANH = 1,
+    EPI_MUL_DTANH = 2,
+    EPI_SOB_T1 = 4,
+    EPI_SOB_K1 = 5,
+    EPI_SOB_K2 = 6,
+    EPI_SOB_A1 = 7
+};
 
-// C = epilogue(opA . opB), opA: M x K, opB: K x N, all FP32 in memory.
+struct Epi {
+    const float* aux = nullptr;
+    const float* aux2 = nullptr;
+    float* out2 = nullptr;
+    const float* vn = nullptr;  // per output column
+    const float* vm = nullptr;  // per output row
+};
+
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+
+__device__ inline bf16x4 to_bf16x4(float a, float b, float c, float d) {
+    return bf16x4{(__bf16)a, (__bf16)b, (__bf16)c, (__bf16)d};
+}
+
+// One 64 (rows of the tile) x kc (K) operand chunk into registers, 8 float4
+// per thread, always along the contiguous memory direction.
+//   KC (K contiguous):   element (r, k) at P[(r0 + r) * ld + k0 + k]
+//   !KC (r contiguous):  element (r, k) at P[(k0 + k) * ld + r0 + r]; a thread
+//                        reads 4 consecutive k rows of one 4-wide r group
+template <bool KC>
+__device__ inline void load_chunk(const float* __restrict__ P, int ld, int r0, int k0, int kc,
+                                  fvec4 (&reg)[8], int tid) {
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+        reg[e] = fvec4{0.f, 0.f, 0.f, 0.f};
+        if (KC) {
+            const int q = tid + 256 * e;
+            const int r = q >> 5, k = (q & 31) * 4;
+            if (k < kc) reg[e] = *(const fvec4*)(P + (size_t)(r0 + r) * ld + k0 + k);
+        } else {
+            const int q = tid + 256 * (e >> 2);
+            const int r = (q & 15) * 4, k = (q >> 4) * 4 + (e & 3);
+            if (k < kc) reg[e] = *(const fvec4*)(P + (size_t)(k0 + k) * ld + r0 + r);
+        }
+    }
+}
+
+// registers of load_chunk -> LDS tile S[r][k] in BF16 (8-byte stores)
+template <bool KC>
+__device__ inline void store_chunk(__bf16 (*S)[kLdsK], int kc, const fvec4 (&reg)[8], int tid) {
+    if (KC) {
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const int q = tid + 256 * e;
+            const int r = q >> 5, k = (q & 31) * 4;
+            if (k < kc) *(bf16x4*)&S[r][k] = to_bf16x4(reg[e][0], reg[e][1], reg[e][2], reg[e][3]);
+        }
+    } else {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int q = tid + 256 * h;
+            const int r = (q & 15) * 4, k = (q >> 4) * 4;
+            if (k < kc) {
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+                    *(bf16x4*)&S[r + i][k] = to_bf16x4(reg[4 * h][i], reg[4 * h + 1][i],
+                                                       reg[4 * h + 2][i], reg[4 * h + 3][i]);
+            }
+        }
+    }
+}
+
+// C = epilogue(opA . opB [+ opA2 . opB2]), opA: M x K, opB: K x N, all FP32 in
+// memory; M, N, K, lda, ldb, ldc are multiples of 64 and the operands 16-byte
+// aligned.  A second segment (A2, B2 with the same shapes and strides, A2 !=
+// nullptr) continues the same accumulation.
 //   opA[m][k] = TA ? A[k*lda + m] : A[m*lda + k]
 //   opB[k][n] = TB ? B[n*ldb + k] : B[k*ldb + n]
-// EPI_STORE: C = acc; EPI_BIAS_TANH: C = tanh(acc + aux[n]);
-// EPI_MUL_DTANH: C = acc * (1 - aux[m*ldc + n]^2); EPI_ACCUM: C += acc.
 // Grid (N/64, M/64), 256 threads: four waves, each a 32 x 32 quarter of the
-// tile as 2 x 2 MFMA 16 x 16 blocks.
+// tile as 2 x 2 MFMA 16 x 16 blocks.  K advances in chunks of 128 staged
+// through LDS as BF16 ([m][k] and [n][k]); the global loads of the next chunk
+// are in flight while the waves run the 4 MFMA steps of the current one.
 template <bool TA, bool TB, int EPI>
-__global__ void __launch_bounds__(256) k_gemm_bf16(const float* __restrict__ A, int lda,
-                                                   const float* __restrict__ B, int ldb,
-                                                   float* __restrict__ C, int ldc, int K,
-                                                   const float* __restrict__ aux) {
-    __shared__ __bf16 As[kTile][kLdsK];  // [m][k]
-    __shared__ __bf16 Bs[kTile][kLdsK];  // [n][k]
+__global__ void __launch_bounds__(256) k_gemm_bf16(const float* __restrict__ A,
+                                                   const float* __restrict__ A2, int lda,
+                                                   const float* __restrict__ B,
+                                                   const float* __restrict__ B2, int ldb,
+                                                   float* __restrict__ C, int ldc, int K, Epi ep) {
+    __shared__ __attribute__((aligned(16))) __bf16 As[kTile][kLdsK];  // [m][k]
+    __shared__ __attribute__((aligned(16))) __bf16 Bs[kTile][kLdsK];  // [n][k]
     const int m0 = blockIdx.y * kTile, n0 = blockIdx.x * kTile;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int wm = (w >> 1) * 32, wn = (w & 1) * 32;
     const int li = lane & 15, lg = lane >> 4;
+    const int nck = (K + kKChunk - 1) / kKChunk;
+    const int nchunks = (A2 ? 2 : 1) * nck;
     fvec4 acc[2][2];
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
         for (int b = 0; b < 2; ++b) acc[a][b] = fvec4{0.f, 0.f, 0.f, 0.f};
-    for (int k0 = 0; k0 < K; k0 += kKStep) {
-        // stage 64 x 32 of opA and of opB^T, 8 elements per thread each; the
-        // fastest index follows the contiguous memory direction
-#pragma unroll
-        for (int e = 0; e < 8; ++e) {
-            const int idx = tid + 256 * e;  // 0 .. 2047
-            const int am = TA ? (idx & 63) : (idx >> 5);
-            const int ak = TA ? (idx >> 6) : (idx & 31);
-            As[am][ak] = (__bf16)(TA ? A[(size_t)(k0 + ak) * lda + m0 + am]
-                                     : A[(size_t)(m0 + am) * lda + k0 + ak]);
-            const int bn = TB ? (idx >> 5) : (idx & 63);
-            const int bk = TB ? (idx & 31) : (idx >> 6);
-            Bs[bn][bk] = (__bf16)(TB ? B[(size_t)(n0 + bn) * ldb + k0 + bk]
-                                     : B[(size_t)(k0 + bk) * ldb + n0 + bn]);
-        }
+    fvec4 ra[8], rb[8];
+    load_chunk<!TA>(A, lda, m0, 0, min(kKChunk, K), ra, tid);
+    load_chunk<TB>(B, ldb, n0, 0, min(kKChunk, K), rb, tid);
+    for (int c = 0; c < nchunks; ++c) {
+        const int k0 = (c % nck) * kKChunk;
+        const int kc = min(kKChunk, K - k0);
+        store_chunk<!TA>(As, kc, ra, tid);
+        store_chunk<TB>(Bs, kc, rb, tid);
         __syncthreads();
-        bf16x8 af[2], bfr[2];
+        if (c + 1 < nchunks) {
+            const bool second = (c + 1) >= nck;
+            const int k1 = ((c + 1) % nck) * kKChunk;
+            const int kc1 = min(kKChunk, K - k1);
+            load_chunk<!TA>(second ? A2 : A, lda, m0, k1, kc1, ra, tid);
+            load_chunk<TB>(second ? B2 : B, ldb, n0, k1, kc1, rb, tid);
+        }
+        for (int kk = 0; kk < kc; kk += kKStep) {
+            bf16x8 af[2], bfr[2];
 #pragma unroll
-        for (int a = 0; a < 2; ++a)
-#pragma unroll
-            for (int e = 0; e < 8; ++e) af[a][e] = As[wm + 16 * a + li][8 * lg + e];
-#pragma unroll
-        for (int b = 0; b < 2; ++b)
-#pragma unroll
-            for (int e = 0; e < 8; ++e) bfr[b][e] = Bs[wn + 16 * b + li][8 * lg + e];
-#pragma unroll
-        for (int a = 0; a < 2; ++a)
+            for (int a = 0; a < 2; ++a)
+                af[a] = *(const bf16x8*)&As[wm + 16 * a + li][kk + 8 * lg];
 #pragma unroll
             for (int b = 0; b < 2; ++b)
-                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[a], bfr[b], acc[a][b], 0, 0, 0);
+                bfr[b] = *(const bf16x8*)&Bs[wn + 16 * b + li][kk + 8 * lg];
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int b = 0; b < 2; ++b)
+                    acc[a][b] =
+                        __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[a], bfr[b], acc[a][b], 0, 0, 0);
+        }
         __syncthreads();
     }
 #pragma unroll
@@ -117,14 +204,32 @@ __global__ void __launch_bounds__(256) k_gemm_bf16(const float* __restrict__ A, 
                 const int m = m0 + wm + 16 * a + 4 * lg + j;
                 const int n = n0 + wn + 16 * b + li;
                 const size_t o = (size_t)m * ldc + n;
-                float v = acc[a][b][j];
-                if (EPI == EPI_BIAS_TANH) v = tanhf(v + aux[n]);
+                const float v = acc[a][b][j];
+                if (EPI == EPI_STORE) C[o] = v;
+                if (EPI == EPI_BIAS_TANH) C[o] = tanhf(v + ep.aux[n]);
                 if (EPI == EPI_MUL_DTANH) {
-                    const float h = aux[o];
-                    v *= 1.f - h * h;
+                    const float h = ep.aux[o];
+                    C[o] = v * (1.f - h * h);
                 }
-                if (EPI == EPI_ACCUM) v += C[o];
-                C[o] = v;
+                if (EPI == EPI_SOB_T1) {
+                    const float h = ep.aux[o];
+                    C[o] = v;
+                    ep.out2[o] = v * (1.f - h * h);
+                }
+                if (EPI == EPI_SOB_K1) {
+                    const float h = ep.aux[o];
+                    C[o] = v * (1.f - h * h);
+                    ep.out2[o] = -2.f * h * v * ep.aux2[o];
+                }
+                if (EPI == EPI_SOB_K2) {
+                    const float h = ep.aux[o], s2 = 1.f - h * h, g = ep.vm[m];
+                    ep.out2[o] = g * ep.aux2[o] - 2.f * h * v * ep.vn[n] * s2;
+                    C[o] = g * h + v * s2;
+                }
+                if (EPI == EPI_SOB_A1) {
+                    const float h = ep.aux[o];
+                    C[o] = (v + ep.aux2[o]) * (1.f - h * h);
+                }
             }
 }
 
@@ -163,88 +268,144 @@ __global__ void __launch_bounds__(256) k_head(const float* __restrict__ h2, int 
     if (lane == 0) y[row] = s + b3[0];
 }
 
-// Loss head of a training batch (n valid rows), one wavefront per row:
-//   e = y - t; dy = e / n; dA2[r][k] = dy w3[k] (1 - h2^2); lrow = e^2 / (2n)
-// padding rows get zeros.
-__global__ void __launch_bounds__(256) k_head_bwd(const float* __restrict__ y,
-                                                  const float* __restrict__ t,
-                                                  const float* __restrict__ h2, int H,
-                                                  const float* __restrict__ w3,
-                                                  float* __restrict__ dy, float* __restrict__ dA2,
-                                                  float* __restrict__ lrow, int n, int rows) {
+// Training head, one wavefront per row (n valid rows, padding rows give zeros):
+//   y = w3 . h2 + b3 + v . z + q . z^2;  e = y - t;  dy = e / n;
+//   lrow = e^2 / (2n);  dA2[r][k] = dy w3[k] (1 - h2^2)
+// With D2 != nullptr (gradient matching) D2[r][k] = w3[k] (1 - h2^2) instead
+// of dA2 (the second backward pass writes dA2).
+__global__ void __launch_bounds__(256) k_head_train(const float* __restrict__ h2, int H,
+                                                    const float* __restrict__ z, int D,
+                                                    const float* __restrict__ w3,
+                                                    const float* __restrict__ b3,
+                                                    const float* __restrict__ v,
+                                                    const float* __restrict__ q,
+                                                    const float* __restrict__ t,
+                                                    float* __restrict__ y, float* __restrict__ dy,
+                                                    float* __restrict__ lrow,
+                                                    float* __restrict__ dA2,
+                                                    float* __restrict__ D2, int n, int rows) {
     const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (row >= rows) return;
-    const float e = row < n ? y[row] - t[row] : 0.f;
+    const float* hr = h2 + (size_t)row * H;
+    const float* zr = z + (size_t)row * D;
+    float s = 0.f;
+    for (int k = lane; k < H; k += 64) s += w3[k] * hr[k];
+    for (int k = lane; k < D; k += 64) {
+        const float zz = zr[k];
+        s += (v[k] + q[k] * zz) * zz;
+    }
+    s = wave_sum_f(s) + b3[0];
+    const float e = row < n ? s - t[row] : 0.f;
     const float g = e / (float)n;
     for (int k = lane; k < H; k += 64) {
-        const float h = h2[(size_t)row * H + k];
-        dA2[(size_t)row * H + k] = g * w3[k] * (1.f - h * h);
+        const float h = hr[k];
+        const float d2 = w3[k] * (1.f - h * h);
+        if (D2)
+            D2[(size_t)row * H + k] = d2;
+        else
+            dA2[(size_t)row * H + k] = g * d2;
     }
     if (lane == 0) {
+        y[row] = s;
         dy[row] = g;
         lrow[row] = 0.5f * e * e / (float)n;
     }
 }
 
-// Column reductions over rows (fixed order, one thread per column):
-//   out[c] = sum_r s[r] X[r][c]         (s == nullptr: s = 1)
-//   out2[c] = sum_r s[r] X[r][c]^2      (out2 != nullptr)
-__global__ void k_colsum(const float* __restrict__ X, int ld, const float* __restrict__ s,
-                         float* __restrict__ out, float* __restrict__ out2, int rows, int cols) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= cols) return;
-    float a = 0.f, a2 = 0.f;
-    for (int r = 0; r < rows; ++r) {
-        const float xv = X[(size_t)r * ld + c];
-        const float sv = s ? s[r] : 1.f;
-        a += sv * xv;
-        a2 += sv * xv * xv;
-    }
-    out[c] = a;
-    if (out2) out2[c] = a2;
-}
+// Column reductions over the rows of a batch, several jobs per launch, each
+// in a fixed order (deterministic):
+//   COL_SUM:  out[c] = sum_r s[r] X[r][c]             (s == nullptr: s = 1)
+//   COL_VQ:   out[c] = sum_r (s[r] X[r][c] + X2[r][c])          (v gradient)
+//             out2[c] = sum_r (s[r] X^2 + 2 X X2)    (X2 == nullptr: 0; q gradient)
+//   COL_SCALAR (one block): out[0] = sum_r s[r]; out2[0] += sum_r X2[r] (if
+//             out2); with a step counter also cur = rates[2 step .. 2 step + 1]
+//             and step + 1 -- the one place a training step advances it.
+// Block: 64 columns (one per lane, coalesced rows) x 16 waves; wave w sums rows
+// w, w + 16, ... and wave 0 adds the 16 partial sums in order.
+constexpr int kColWaves = 16;
+enum { COL_SUM = 0, COL_VQ = 1, COL_SCALAR = 2 };
 
-// gb3 = sum dy; loss_acc += sum lrow (single wavefront)
-__global__ void k_sum2(const float* __restrict__ dy, const float* __restrict__ lrow,
-                       float* __restrict__ gb3, float* __restrict__ loss_acc, int rows) {
+struct ColJob {
+    int kind = COL_SUM;
+    int cols = 0;
+    int block0 = 0;  // first block of the job
+    const float* X = nullptr;
+    const float* s = nullptr;
+    const float* X2 = nullptr;
+    float* out = nullptr;
+    float* out2 = nullptr;
+};
+
+struct ColJobs {
+    ColJob job[4];
+    int njobs = 0;
+    int rows = 0;
+    int32_t* step = nullptr;
+    const float* rates = nullptr;
+    float* cur = nullptr;
+};
+
+__global__ void __launch_bounds__(64 * kColWaves) k_colsums(ColJobs jobs) {
+    __shared__ float part[2][kColWaves][64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int j = 0;
+    while (j + 1 < jobs.njobs && (int)blockIdx.x >= jobs.job[j + 1].block0) ++j;
+    const ColJob& jb = jobs.job[j];
+    const int rows = jobs.rows;
     float a = 0.f, b = 0.f;
-    for (int i = threadIdx.x; i < rows; i += 64) {
-        a += dy[i];
-        b += lrow[i];
+    if (jb.kind == COL_SCALAR) {
+        for (int r = w * 64 + lane; r < rows; r += 64 * kColWaves) {
+            a += jb.s[r];
+            if (jb.out2) b += jb.X2[r];
+        }
+        a = wave_sum_f(a);
+        b = wave_sum_f(b);
+    } else {
+        const int c = ((int)blockIdx.x - jb.block0) * 64 + lane;
+        if (c < jb.cols) {
+            const int ld = jb.cols;
+#pragma unroll 4
+            for (int r = w; r < rows; r += kColWaves) {
+                const float xv = jb.X[(size_t)r * ld + c];
+                const float sv = jb.s ? jb.s[r] : 1.f;
+                if (jb.kind == COL_SUM) {
+                    a += sv * xv;
+                } else {
+                    const float p = jb.X2 ? jb.X2[(size_t)r * ld + c] : 0.f;
+                    a += sv * xv + p;
+                    b += sv * xv * xv + 2.f * xv * p;
+                }
+            }
+        }
     }
-    a = wave_sum_f(a);
-    b = wave_sum_f(b);
-    if (threadIdx.x == 0) {
-        gb3[0] = a;
-        if (loss_acc) loss_acc[0] += b;
+    part[0][w][lane] = a;
+    part[1][w][lane] = b;
+    __syncthreads();
+    if (w != 0) return;
+    float t = 0.f, t2 = 0.f;
+    for (int k = 0; k < kColWaves; ++k) {
+        t += part[0][k][lane];
+        t2 += part[1][k][lane];
     }
-}
-
-// Adam with the bias corrections folded into lr_t = lr sqrt(1-b2^t)/(1-b1^t)
-// and eps_t = eps sqrt(1-b2^t)
-__global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
-                       float* __restrict__ v, long long n, float lr_t, float b1, float b2,
-                       float eps_t) {
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const float gi = g[i];
-    const float mi = b1 * m[i] + (1.f - b1) * gi;
-    const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
-    m[i] = mi;
-    v[i] = vi;
-    p[i] -= lr_t * mi / (sqrtf(vi) + eps_t);
-}
-
-// minibatch rows from the resident training set (padding rows zero)
-__global__ void k_gather_rows(const float* __restrict__ Zall, const int32_t* __restrict__ idx,
-                              const float* __restrict__ tall, float* __restrict__ Z,
-                              float* __restrict__ t, int n, int rows, int D) {
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (long long)rows * D) return;
-    const int r = (int)(i / D), c = (int)(i % D);
-    Z[i] = r < n ? Zall[(size_t)idx[r] * D + c] : 0.f;
-    if (c == 0) t[r] = r < n ? tall[idx[r]] : 0.f;
+    if (jb.kind == COL_SCALAR) {
+        if (lane == 0) {
+            jb.out[0] = t;
+            if (jb.out2) jb.out2[0] += t2;
+            if (jobs.step) {
+                const int st = jobs.step[0];
+                jobs.cur[0] = jobs.rates[2 * st];
+                jobs.cur[1] = jobs.rates[2 * st + 1];
+                jobs.step[0] = st + 1;
+            }
+        }
+    } else {
+        const int c = ((int)blockIdx.x - jb.block0) * 64 + lane;
+        if (c < jb.cols) {
+            jb.out[c] = t;
+            if (jb.kind == COL_VQ) jb.out2[c] = t2;
+        }
+    }
 }
 
 // Z[r][c] = (log x[r][c] - mean[c]) / sd[c] for r < n, c < d; zero padding
@@ -284,7 +445,8 @@ __global__ void k_finalize(const float* __restrict__ y, const float* __restrict_
 // with g = dy/dz = W1^T delta1 + v + 2 q z (the input gradient, delta2 =
 // w3 (1 - h2^2), delta1 = (W2^T delta2)(1 - h1^2)) and G the standardised
 // exact gradient.  Its parameter gradient is a second backward pass through
-// the input-gradient graph; the kernels below are its elementwise steps.
+// the input-gradient graph; its elementwise steps are the EPI_SOB_* GEMM
+// epilogues and the row kernel below.
 
 // rho[r][c] = lam_n (dZ[r][c] + v[c] + 2 q[c] z[r][c] - G[r][c]) for r < n,
 // c < d (zero elsewhere), lam_n = lambda / (n d); lrow[r] += lam_n/2 sum_c
@@ -311,75 +473,46 @@ __global__ void __launch_bounds__(256) k_sob_rho(const float* __restrict__ dZ,
     if (lane == 0) lrow[row] += 0.5f * lam_n * acc;
 }
 
-// Dl1 = T1 (1 - h1^2)  (delta1 from T1 = delta2 W2)
-__global__ void k_sob_delta1(const float* __restrict__ T1, const float* __restrict__ H1,
-                             float* __restrict__ Dl1, long long total) {
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= total) return;
-    const float h = H1[i];
-    Dl1[i] = T1[i] * (1.f - h * h);
-}
+// ---- graph-replayed training steps -----------------------------------------
+// The minibatch offset and Adam's per-step rate come from device memory
+// (a step counter and a per-epoch table), so one captured step serves every
+// minibatch of an epoch.
 
-// from kappa1 = rho W1^T (in K1):  K1 <- mu1 = kappa1 (1 - h1^2);
-// Gam1 = -2 h1 kappa1 T1
-__global__ void k_sob_h1(float* __restrict__ K1, const float* __restrict__ T1,
-                         const float* __restrict__ H1, float* __restrict__ Gam1, long long total) {
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= total) return;
-    const float h = H1[i], k1 = K1[i];
-    K1[i] = k1 * (1.f - h * h);
-    Gam1[i] = -2.f * h * k1 * T1[i];
-}
-
-// from kappa2 = mu1 W2^T (in K2), per row r with dy[r]:
-//   A2tot = dy delta2 - 2 h2 kappa2 w3 (1 - h2^2)      -> dA2
-//   K2   <- dy h2 + kappa2 (1 - h2^2)                  (rows of the w3 gradient)
-__global__ void k_sob_h2(float* __restrict__ K2, const float* __restrict__ H2,
-                         const float* __restrict__ D2, const float* __restrict__ w3,
-                         const float* __restrict__ dy, float* __restrict__ dA2, int H,
-                         long long total) {
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= total) return;
-    const int c = (int)(i % H);
-    const float g = dy[i / H];
-    const float h = H2[i], k2 = K2[i], s2 = 1.f - h * h;
-    dA2[i] = g * D2[i] - 2.f * h * k2 * w3[c] * s2;
-    K2[i] = g * h + k2 * s2;
-}
-
-// A1 <- (A1 + Gam1) (1 - h1^2)   (A1 = A2tot W2 on entry)
-__global__ void k_sob_a1(float* __restrict__ A1, const float* __restrict__ Gam1,
-                         const float* __restrict__ H1, long long total) {
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= total) return;
-    const float h = H1[i];
-    A1[i] = (A1[i] + Gam1[i]) * (1.f - h * h);
-}
-
-// gv[c] = sum_r dy_r z + rho;  gq[c] = sum_r dy_r z^2 + 2 z rho  (fixed order)
-__global__ void k_colsum_vq(const float* __restrict__ Z, const float* __restrict__ dy,
-                            const float* __restrict__ rho, float* __restrict__ gv,
-                            float* __restrict__ gq, int rows, int D) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= D) return;
-    float a = 0.f, b = 0.f;
-    for (int r = 0; r < rows; ++r) {
-        const float z = Z[(size_t)r * D + c], g = dy[r];
-        const float p = rho ? rho[(size_t)r * D + c] : 0.f;
-        a += g * z + p;
-        b += g * z * z + 2.f * z * p;
-    }
-    gv[c] = a;
-    gq[c] = b;
-}
-
-// minibatch rows of the exact-gradient targets (padding rows zero)
-__global__ void k_gather_grad(const float* __restrict__ Gall, const int32_t* __restrict__ idx,
-                              float* __restrict__ G, int n, int rows, int D) {
+// rows of minibatch *step (size `batch`) from the resident set (padding zero)
+__global__ void k_gather_step(const float* __restrict__ Zall, const int32_t* __restrict__ idx,
+                              const int32_t* __restrict__ step, int batch,
+                              const float* __restrict__ tall, const float* __restrict__ Gall,
+                              float* __restrict__ Z, float* __restrict__ t, float* __restrict__ G,
+                              int n, int rows, int D) {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (long long)rows * D) return;
     const int r = (int)(i / D), c = (int)(i % D);
-    G[i] = r < n ? Gall[(size_t)idx[r] * D + c] : 0.f;
+    const int32_t* id = idx + (size_t)step[0] * batch;
+    const bool valid = r < n;
+    const size_t src = valid ? (size_t)id[r] * D + c : 0;
+    Z[i] = valid ? Zall[src] : 0.f;
+    if (G) G[i] = valid ? Gall[src] : 0.f;
+    if (c == 0) t[r] = valid ? tall[id[r]] : 0.f;
+}
+
+// Adam with the bias corrections folded into lr_t = lr sqrt(1-b2^t)/(1-b1^t)
+// and eps_t = eps sqrt(1-b2^t); (lr_t, eps_t) = cur[0], cur[1] (device memory)
+__global__ void k_adam_step(float* __restrict__ p, const float* __restrict__ g,
+                            float* __restrict__ m, float* __restrict__ v, long long n,
+                            const float* __restrict__ cur, float b1, float b2) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float lr_t = cur[0], eps_t = cur[1];
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.f - b1) * gi;
+    const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    p[i] -= lr_t * mi / (sqrtf(vi) + eps_t);
+}
+
+__global__ void k_step_set(int32_t* __restrict__ step, int value) {
+    if (threadIdx.x == 0) step[0] = value;
 }
 
 int pad_to(int x, int m) { return (x + m - 1) / m * m; }
@@ -415,6 +548,15 @@ struct mcp_mlp {
     float *Zall = nullptr, *tall = nullptr, *Gall = nullptr;
     int32_t* idx = nullptr;
     int64_t ntrain = 0;
+    // graph-replayed training step
+    int32_t* dstep = nullptr;
+    float* dcur = nullptr;    // the current step's (lr_t, eps_t)
+    float* drates = nullptr;  // [2 * steps per epoch] (lr_t, eps_t)
+    size_t drates_cap = 0;
+    hipGraphExec_t train_graph = nullptr;
+    int train_graph_batch = 0;
+    bool train_graph_sob = false;
+    bool train_graph_on = true;
     // surrogate leapfrog
     int chains = 0;
     double prior_scale = -1.0;
@@ -428,13 +570,18 @@ struct mcp_mlp {
     void drop_graph() {
         if (lf_graph) (void)hipGraphExecDestroy(lf_graph);
         lf_graph = nullptr;
+        drop_train_graph();
+    }
+    void drop_train_graph() {
+        if (train_graph) (void)hipGraphExecDestroy(train_graph);
+        train_graph = nullptr;
     }
     ~mcp_mlp() {
         if (stream) (void)hipStreamSynchronize(stream);
         drop_graph();
         void* ptrs[] = {p,   gp,  am,  av,    Z,   H1,   H2,   y,   t,      dy,  lrow,   dA2,
                         dH1, dZ,  dX,  dOut,  loss, xmean, xsd, Zall, tall, idx, dstate, dlp,
-                        ditems, Gb, rho, D2, T1, Dl1, K1, K2, Gam1, Gall};
+                        ditems, Gb, rho, D2, T1, Dl1, K1, K2, Gam1, Gall, dstep, dcur, drates};
         for (void* ptr : ptrs)
             if (ptr) (void)hipFree(ptr);
         if (hstate) (void)hipHostFree(hstate);
@@ -506,9 +653,31 @@ int ensure_rows(mcp_mlp* m, int need) {
 
 template <bool TA, bool TB, int EPI>
 void gemm(mcp_mlp* m, int M, int N, int K, const float* A, int lda, const float* B, int ldb,
-          float* C, int ldc, const float* aux) {
+          float* C, int ldc, Epi ep = Epi{}, const float* A2 = nullptr,
+          const float* B2 = nullptr) {
     hipLaunchKernelGGL((k_gemm_bf16<TA, TB, EPI>), dim3(N / kTile, M / kTile), dim3(256), 0,
-                       m->stream, A, lda, B, ldb, C, ldc, K, aux);
+                       m->stream, A, A2, lda, B, B2, ldb, C, ldc, K, ep);
+}
+
+Epi epi(const float* aux, const float* aux2 = nullptr, float* out2 = nullptr,
+        const float* vn = nullptr, const float* vm = nullptr) {
+    Epi e;
+    e.aux = aux;
+    e.aux2 = aux2;
+    e.out2 = out2;
+    e.vn = vn;
+    e.vm = vm;
+    return e;
+}
+
+// h1 = tanh(W1 z + b1), h2 = tanh(W2 h1 + b2) over `rows` rows of Z
+void forward_hidden(mcp_mlp* m, int rows) {
+    const int H = m->H, D = m->D;
+    const float* P = m->p;
+    gemm<false, true, EPI_BIAS_TANH>(m, rows, H, D, m->Z, D, P + m->oW1, D, m->H1, H,
+                                     epi(P + m->ob1));
+    gemm<false, true, EPI_BIAS_TANH>(m, rows, H, H, m->H1, H, P + m->oW2, H, m->H2, H,
+                                     epi(P + m->ob2));
 }
 
 // h1, h2, y over `rows` rows of Z; with `seed` also the input-gradient seed
@@ -516,8 +685,7 @@ void gemm(mcp_mlp* m, int M, int N, int K, const float* A, int lda, const float*
 void forward(mcp_mlp* m, int rows, float* seed) {
     const int H = m->H, D = m->D;
     const float* P = m->p;
-    gemm<false, true, EPI_BIAS_TANH>(m, rows, H, D, m->Z, D, P + m->oW1, D, m->H1, H, P + m->ob1);
-    gemm<false, true, EPI_BIAS_TANH>(m, rows, H, H, m->H1, H, P + m->oW2, H, m->H2, H, P + m->ob2);
+    forward_hidden(m, rows);
     hipLaunchKernelGGL(k_head, dim3((rows + 3) / 4), dim3(256), 0, m->stream, m->H2, H, m->Z, D,
                        P + m->ow3, P + m->ob3, P + m->ov, P + m->oq, m->y, seed, rows);
 }
@@ -525,61 +693,108 @@ void forward(mcp_mlp* m, int rows, float* seed) {
 // input gradient dZ = ((dA2 W2) * (1 - h1^2)) W1 (without the skip terms)
 void backward_input(mcp_mlp* m, int rows) {
     const int H = m->H, D = m->D;
-    gemm<false, false, EPI_MUL_DTANH>(m, rows, H, H, m->dA2, H, m->p + m->oW2, H, m->dH1, H, m->H1);
-    gemm<false, false, EPI_STORE>(m, rows, D, H, m->dH1, H, m->p + m->oW1, D, m->dZ, D, nullptr);
+    gemm<false, false, EPI_MUL_DTANH>(m, rows, H, H, m->dA2, H, m->p + m->oW2, H, m->dH1, H,
+                                      epi(m->H1));
+    gemm<false, false, EPI_STORE>(m, rows, D, H, m->dH1, H, m->p + m->oW1, D, m->dZ, D);
 }
 
-// parameter gradients of the batch loss into gp (every entry written); with
-// lam > 0 the gradient-matching term against the targets in Gb is included
-// (needs delta2 in D2 from forward()).
-void backward_params(mcp_mlp* m, int rows, int n, float* loss_acc, float lam) {
+ColJob coljob(int kind, int cols, const float* X, const float* s, const float* X2, float* out,
+              float* out2) {
+    ColJob j;
+    j.kind = kind;
+    j.cols = cols;
+    j.X = X;
+    j.s = s;
+    j.X2 = X2;
+    j.out = out;
+    j.out2 = out2;
+    return j;
+}
+
+void launch_colsums(mcp_mlp* m, ColJobs& jobs) {
+    int blocks = 0;
+    for (int k = 0; k < jobs.njobs; ++k) {
+        jobs.job[k].block0 = blocks;
+        blocks += jobs.job[k].kind == COL_SCALAR ? 1 : (jobs.job[k].cols + 63) / 64;
+    }
+    hipLaunchKernelGGL(k_colsums, dim3(blocks), dim3(64 * kColWaves), 0, m->stream, jobs);
+}
+
+// Parameter gradients of the batch loss into gp (every entry written) from
+// h1, h2 of forward_hidden(); with lam > 0 the gradient-matching term against
+// the targets in Gb is included.  loss_acc += batch loss.  With `advance` the
+// scalar column job also moves the training step counter and fetches the
+// step's Adam rates into dcur.
+void backward_params(mcp_mlp* m, int rows, int n, float* loss_acc, float lam, bool advance) {
     const int H = m->H, D = m->D;
-    const float* W1 = m->p + m->oW1;
-    const float* W2 = m->p + m->oW2;
+    const float* P = m->p;
+    const float* W1 = P + m->oW1;
+    const float* W2 = P + m->oW2;
     float* G = m->gp;
     hipStream_t st = m->stream;
-    const long long tH = (long long)rows * H;
-    const unsigned bH = nblocks(tH);
-    hipLaunchKernelGGL(k_head_bwd, dim3((rows + 3) / 4), dim3(256), 0, st, m->y, m->t, m->H2, H,
-                       m->p + m->ow3, m->dy, m->dA2, m->lrow, n, rows);
-    if (lam > 0.f) {
-        // input gradient g = W1^T delta1 + v + 2 q z
-        gemm<false, false, EPI_STORE>(m, rows, H, H, m->D2, H, W2, H, m->T1, H, nullptr);
-        hipLaunchKernelGGL(k_sob_delta1, dim3(bH), dim3(256), 0, st, m->T1, m->H1, m->Dl1, tH);
-        gemm<false, false, EPI_STORE>(m, rows, D, H, m->Dl1, H, W1, D, m->dZ, D, nullptr);
+    const bool sob = lam > 0.f;
+    hipLaunchKernelGGL(k_head_train, dim3((rows + 3) / 4), dim3(256), 0, st, m->H2, H, m->Z, D,
+                       P + m->ow3, P + m->ob3, P + m->ov, P + m->oq, m->t, m->y, m->dy, m->lrow,
+                       m->dA2, sob ? m->D2 : nullptr, n, rows);
+    if (sob) {
+        // input gradient g = W1^T delta1 + v + 2 q z (T1 = delta2 W2, delta1)
+        gemm<false, false, EPI_SOB_T1>(m, rows, H, H, m->D2, H, W2, H, m->T1, H,
+                                       epi(m->H1, nullptr, m->Dl1));
+        gemm<false, false, EPI_STORE>(m, rows, D, H, m->Dl1, H, W1, D, m->dZ, D);
         hipLaunchKernelGGL(k_sob_rho, dim3((rows + 3) / 4), dim3(256), 0, st, m->dZ, m->Z, m->Gb,
-                           m->p + m->ov, m->p + m->oq, m->rho, m->lrow, D, m->d, n, rows,
+                           P + m->ov, P + m->oq, m->rho, m->lrow, D, m->d, n, rows,
                            lam / ((float)n * (float)m->d));
         // second backward pass through the input-gradient graph
-        gemm<false, true, EPI_STORE>(m, rows, H, D, m->rho, D, W1, D, m->K1, H, nullptr);
-        hipLaunchKernelGGL(k_sob_h1, dim3(bH), dim3(256), 0, st, m->K1, m->T1, m->H1, m->Gam1, tH);
-        gemm<false, true, EPI_STORE>(m, rows, H, H, m->K1, H, W2, H, m->K2, H, nullptr);
-        hipLaunchKernelGGL(k_sob_h2, dim3(bH), dim3(256), 0, st, m->K2, m->H2, m->D2, m->p + m->ow3,
-                           m->dy, m->dA2, H, tH);
-        hipLaunchKernelGGL(k_colsum, dim3(nblocks(H)), dim3(256), 0, st, m->K2, H, nullptr,
-                           G + m->ow3, nullptr, rows, H);
-    } else {
-        hipLaunchKernelGGL(k_colsum, dim3(nblocks(H)), dim3(256), 0, st, m->H2, H, m->dy,
-                           G + m->ow3, nullptr, rows, H);
+        gemm<false, true, EPI_SOB_K1>(m, rows, H, D, m->rho, D, W1, D, m->K1, H,
+                                      epi(m->H1, m->T1, m->Gam1));
+        gemm<false, true, EPI_SOB_K2>(m, rows, H, H, m->K1, H, W2, H, m->K2, H,
+                                      epi(m->H2, m->D2, m->dA2, P + m->ow3, m->dy));
     }
-    hipLaunchKernelGGL(k_sum2, dim3(1), dim3(64), 0, st, m->dy, m->lrow, G + m->ob3, loss_acc, rows);
-    hipLaunchKernelGGL(k_colsum_vq, dim3(nblocks(D)), dim3(256), 0, st, m->Z, m->dy,
-                       lam > 0.f ? m->rho : nullptr, G + m->ov, G + m->oq, rows, D);
-    hipLaunchKernelGGL(k_colsum, dim3(nblocks(H)), dim3(256), 0, st, m->dA2, H, nullptr, G + m->ob2,
-                       nullptr, rows, H);
-    gemm<true, false, EPI_STORE>(m, H, H, rows, m->dA2, H, m->H1, H, G + m->oW2, H, nullptr);
-    if (lam > 0.f) {
-        gemm<true, false, EPI_ACCUM>(m, H, H, rows, m->D2, H, m->K1, H, G + m->oW2, H, nullptr);
-        gemm<false, false, EPI_STORE>(m, rows, H, H, m->dA2, H, W2, H, m->dH1, H, nullptr);
-        hipLaunchKernelGGL(k_sob_a1, dim3(bH), dim3(256), 0, st, m->dH1, m->Gam1, m->H1, tH);
-    } else {
-        gemm<false, false, EPI_MUL_DTANH>(m, rows, H, H, m->dA2, H, W2, H, m->dH1, H, m->H1);
+    ColJobs jobs;
+    jobs.rows = rows;
+    jobs.job[0] = sob ? coljob(COL_SUM, H, m->K2, nullptr, nullptr, G + m->ow3, nullptr)
+                      : coljob(COL_SUM, H, m->H2, m->dy, nullptr, G + m->ow3, nullptr);
+    jobs.job[1] = coljob(COL_SUM, H, m->dA2, nullptr, nullptr, G + m->ob2, nullptr);
+    jobs.job[2] = coljob(COL_VQ, D, m->Z, m->dy, sob ? m->rho : nullptr, G + m->ov, G + m->oq);
+    jobs.job[3] = coljob(COL_SCALAR, 1, nullptr, m->dy, m->lrow, G + m->ob3, loss_acc);
+    jobs.njobs = 4;
+    if (advance) {
+        jobs.step = m->dstep;
+        jobs.rates = m->drates;
+        jobs.cur = m->dcur;
     }
-    hipLaunchKernelGGL(k_colsum, dim3(nblocks(H)), dim3(256), 0, st, m->dH1, H, nullptr, G + m->ob1,
-                       nullptr, rows, H);
-    gemm<true, false, EPI_STORE>(m, H, D, rows, m->dH1, H, m->Z, D, G + m->oW1, D, nullptr);
-    if (lam > 0.f)
-        gemm<true, false, EPI_ACCUM>(m, H, D, rows, m->Dl1, H, m->rho, D, G + m->oW1, D, nullptr);
+    launch_colsums(m, jobs);
+    // gW2 = A2tot^T h1 (+ delta2^T mu1)
+    gemm<true, false, EPI_STORE>(m, H, H, rows, m->dA2, H, m->H1, H, G + m->oW2, H, Epi{},
+                                 sob ? m->D2 : nullptr, sob ? m->K1 : nullptr);
+    if (sob)
+        gemm<false, false, EPI_SOB_A1>(m, rows, H, H, m->dA2, H, W2, H, m->dH1, H,
+                                       epi(m->H1, m->Gam1));
+    else
+        gemm<false, false, EPI_MUL_DTANH>(m, rows, H, H, m->dA2, H, W2, H, m->dH1, H, epi(m->H1));
+    ColJobs jb1;
+    jb1.rows = rows;
+    jb1.job[0] = coljob(COL_SUM, H, m->dH1, nullptr, nullptr, G + m->ob1, nullptr);
+    jb1.njobs = 1;
+    launch_colsums(m, jb1);
+    // gW1 = A1tot^T z (+ delta1^T rho)
+    gemm<true, false, EPI_STORE>(m, H, D, rows, m->dH1, H, m->Z, D, G + m->oW1, D, Epi{},
+                                 sob ? m->Dl1 : nullptr, sob ? m->rho : nullptr);
+}
+
+// one Adam step on minibatch *dstep (nb rows): gather, forward, backward
+// (which advances the step counter), update.  Every launch reads the step
+// or its rates from device memory.
+void enqueue_train_step(mcp_mlp* m, int batch, int nb, float lam) {
+    const int rows = pad_to(nb, kTile);
+    hipLaunchKernelGGL(k_gather_step, dim3(nblocks((long long)rows * m->D)), dim3(256), 0,
+                       m->stream, m->Zall, m->idx, m->dstep, batch, m->tall,
+                       lam > 0.f ? m->Gall : nullptr, m->Z, m->t, lam > 0.f ? m->Gb : nullptr,
+                       nb, rows, m->D);
+    forward_hidden(m, rows);
+    backward_params(m, rows, nb, m->loss, lam, true);
+    hipLaunchKernelGGL(k_adam_step, dim3(nblocks((long long)m->nparam)), dim3(256), 0, m->stream,
+                       m->p, m->gp, m->am, m->av, (long long)m->nparam, m->dcur, 0.9f, 0.999f);
 }
 
 // logL~ and its gradient for the n rows of m->dX (already on the device)
@@ -782,6 +997,7 @@ int mcp_mlp_set_data(mcp_mlp* m, const double* X, const double* logl, const doub
         th[i] = (float)((logl[i] - (double)m->ymean) / (double)m->ysd);
     }
     MLP_TRY(hipStreamSynchronize(m->stream));
+    m->drop_train_graph();  // the captured step holds the old buffer addresses
     float** fb[] = {&m->Zall, &m->tall, &m->Gall};
     for (float** q : fb) {
         if (*q) (void)hipFree(*q);
@@ -820,33 +1036,60 @@ int mcp_mlp_train(mcp_mlp* m, int32_t batch, int32_t epochs, float lr, float lr_
     std::mt19937_64 rng(seed);
     std::vector<int32_t> perm(n);
     for (int64_t i = 0; i < n; ++i) perm[i] = (int32_t)i;
-    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
-    const int64_t nbatch = (n + batch - 1) / batch;
+    const double b1 = 0.9, b2 = 0.999, eps = 1e-8;
+    const int64_t nfull = n / batch;
+    const int rem = (int)(n % batch);
+    const int64_t nbatch = nfull + (rem ? 1 : 0);
+    if (!m->dstep) MLP_TRY(hipMalloc(&m->dstep, 64 * sizeof(int32_t)));
+    if (!m->dcur) MLP_TRY(hipMalloc(&m->dcur, 64 * sizeof(float)));
+    if (m->drates_cap < (size_t)(2 * nbatch)) {
+        m->drop_train_graph();
+        if (m->drates) (void)hipFree(m->drates);
+        m->drates = nullptr;
+        m->drates_cap = 0;
+        MLP_TRY(hipMalloc(&m->drates, (size_t)(2 * nbatch) * sizeof(float)));
+        m->drates_cap = (size_t)(2 * nbatch);
+    }
+    if (m->train_graph && (m->train_graph_batch != batch || m->train_graph_sob != (lam > 0.f)))
+        m->drop_train_graph();
+    std::vector<float> rates((size_t)(2 * nbatch));
     for (int ep = 0; ep < epochs; ++ep) {
         // exponential decay from lr to lr_final over the epochs
         const double frac = epochs > 1 ? (double)ep / (double)(epochs - 1) : 0.0;
         const double lr_e = (double)lr * std::pow((double)lr_final / (double)lr, frac);
+        for (int64_t k = 0; k < nbatch; ++k) {
+            const double tt = (double)(m->adam_t + k + 1);
+            const double c1 = 1.0 - std::pow(b1, tt), c2 = 1.0 - std::pow(b2, tt);
+            rates[2 * k] = (float)(lr_e * std::sqrt(c2) / c1);
+            rates[2 * k + 1] = (float)(eps * std::sqrt(c2));
+        }
         std::shuffle(perm.begin(), perm.end(), rng);
         MLP_TRY(h2d(m, m->idx, perm.data(), n * sizeof(int32_t)));
+        MLP_TRY(h2d(m, m->drates, rates.data(), rates.size() * sizeof(float)));
         MLP_TRY(hipMemsetAsync(m->loss, 0, sizeof(float), m->stream));
-        for (int64_t b0 = 0; b0 < n; b0 += batch) {
-            const int nb = (int)std::min<int64_t>(batch, n - b0);
-            const int rows = pad_to(nb, kTile);
-            hipLaunchKernelGGL(k_gather_rows, dim3(nblocks((long long)rows * m->D)), dim3(256), 0,
-                               m->stream, m->Zall, m->idx + b0, m->tall, m->Z, m->t, nb, rows, m->D);
-            if (lam > 0.f)
-                hipLaunchKernelGGL(k_gather_grad, dim3(nblocks((long long)rows * m->D)), dim3(256),
-                                   0, m->stream, m->Gall, m->idx + b0, m->Gb, nb, rows, m->D);
-            forward(m, rows, lam > 0.f ? m->D2 : nullptr);
-            backward_params(m, rows, nb, m->loss, lam);
-            ++m->adam_t;
-            const double c1 = 1.0 - std::pow((double)b1, (double)m->adam_t);
-            const double c2 = 1.0 - std::pow((double)b2, (double)m->adam_t);
-            const float lr_t = (float)(lr_e * std::sqrt(c2) / c1);
-            const float eps_t = (float)(eps * std::sqrt(c2));
-            hipLaunchKernelGGL(k_adam, dim3(nblocks((long long)m->nparam)), dim3(256), 0, m->stream,
-                               m->p, m->gp, m->am, m->av, (long long)m->nparam, lr_t, b1, b2, eps_t);
+        hipLaunchKernelGGL(k_step_set, dim3(1), dim3(64), 0, m->stream, m->dstep, 0);
+        if (nfull > 0 && !m->train_graph_on) {
+            for (int64_t k = 0; k < nfull; ++k) enqueue_train_step(m, batch, batch, lam);
+        } else if (nfull > 0) {
+            if (!m->train_graph) {
+                MLP_TRY(hipStreamBeginCapture(m->stream, hipStreamCaptureModeThreadLocal));
+                enqueue_train_step(m, batch, batch, lam);
+                hipGraph_t cap = nullptr;
+                const hipError_t ce = hipStreamEndCapture(m->stream, &cap);
+                if (ce != hipSuccess) return set_last_error(MCP_ERR_HIP, "graph capture failed");
+                const hipError_t ie = hipGraphInstantiate(&m->train_graph, cap, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(cap);
+                if (ie != hipSuccess) {
+                    m->train_graph = nullptr;
+                    return set_last_error(MCP_ERR_HIP, "graph instantiation failed");
+                }
+                m->train_graph_batch = batch;
+                m->train_graph_sob = lam > 0.f;
+            }
+            for (int64_t k = 0; k < nfull; ++k) MLP_TRY(hipGraphLaunch(m->train_graph, m->stream));
         }
+        if (rem) enqueue_train_step(m, batch, rem, lam);
+        m->adam_t += nbatch;
         MLP_TRY(hipGetLastError());
         if (epoch_loss) {
             float l = 0.f;
@@ -883,8 +1126,8 @@ int mcp_mlp_param_grad(mcp_mlp* m, const double* X, const double* logl, const do
         MLP_TRY(h2d(m, m->Gb, Gh.data(), Gh.size() * sizeof(float)));
     }
     MLP_TRY(hipMemsetAsync(m->loss, 0, sizeof(float), m->stream));
-    forward(m, rows, lam > 0.f ? m->D2 : nullptr);
-    backward_params(m, rows, (int)n, m->loss, lam);
+    forward_hidden(m, rows);
+    backward_params(m, rows, (int)n, m->loss, lam, false);
     MLP_TRY(hipGetLastError());
     MLP_TRY(d2h(m, grad, m->gp, m->nparam * sizeof(float)));
     if (loss) MLP_TRY(d2h(m, loss, m->loss, sizeof(float)));
@@ -1001,6 +1244,15 @@ int mcp_mlp_use_graph(mcp_mlp* m, int32_t on) {
     MLP_TRY(hipStreamSynchronize(m->stream));
     m->use_graph = on != 0;
     if (!m->use_graph) m->drop_graph();
+    return MCP_OK;
+}
+
+int mcp_mlp_train_graph(mcp_mlp* m, int32_t on) {
+    if (!m) return set_last_error(MCP_ERR_INVALID, "null argument");
+    DeviceGuard guard(m->device);
+    MLP_TRY(hipStreamSynchronize(m->stream));
+    m->train_graph_on = on != 0;
+    if (!m->train_graph_on) m->drop_train_graph();
     return MCP_OK;
 }
 

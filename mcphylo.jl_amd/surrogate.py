@@ -159,6 +159,10 @@ class Surrogate:
     def use_graph(self, on: bool = True) -> None:
         check(_lib.load_library().mcp_mlp_use_graph(self._h, 1 if on else 0))
 
+    def train_graph(self, on: bool = True) -> None:
+        """Replay one captured graph per full training minibatch (default)."""
+        check(_lib.load_library().mcp_mlp_train_graph(self._h, 1 if on else 0))
+
     def leapfrog(self, eps_delta, x, r, g, prior_scale: float | None = None) -> np.ndarray:
         """One surrogate leapfrog for every chain; x, r, g ([chains][d],
         C-contiguous float64) are updated in place; returns the log posterior."""

@@ -112,6 +112,32 @@ def test_training_fits_a_smooth_target_and_is_deterministic():
     assert r2 > 0.95, r2
 
 
+@pytest.mark.parametrize("lam", [0.0, 3.0])
+def test_training_graph_replay_matches_eager_steps(lam):
+    # n = 3 full minibatches + a remainder: the graph serves the full ones
+    # (offset and Adam rate from the device step counter), the remainder runs
+    # eagerly; turning the graph off must give the same bits.
+    d, H, n, batch = 21, 128, 3 * 128 + 37, 128
+    rng = np.random.default_rng(9)
+    X = np.exp(rng.normal(-2.0, 0.5, (n, d)))
+    y = -200.0 - np.sum(np.log(X) ** 2, axis=1)
+    G = -2.0 * np.log(X) / X
+    runs = []
+    for graph in (True, False):
+        s = mcp.Surrogate(d, H, 0, 4)
+        s.train_graph(graph)
+        s.fit_scaling(X, y)
+        s.set_data(X, y, G)
+        l1 = s.train(3, batch=batch, lr=2e-3, lr_final=1e-4, seed=8, grad_weight=lam)
+        # a second call continues the Adam step count and reuses the graph
+        l2 = s.train(2, batch=batch, lr=1e-3, lr_final=1e-4, seed=9, grad_weight=lam)
+        runs.append((s.get_flat(), l1, l2))
+    (p1, a1, b1), (p2, a2, b2) = runs
+    assert np.all(np.isfinite(p1)) and a1[-1] < a1[0]
+    assert np.array_equal(a1, a2) and np.array_equal(b1, b2)
+    assert np.array_equal(p1, p2)
+
+
 def test_surrogate_leapfrog_matches_host_restatement():
     d, H = 17, 256
     s, flat, X, rng = _random_surrogate(d, H, 99)
