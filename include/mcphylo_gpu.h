@@ -150,6 +150,24 @@ int mcp_logl_grad_by_rate(mcp_plan* plan, const double* blv, double* ll, double*
 int mcp_batch_logl_grad_by_rate(mcp_batch* batch, const double* blv_concat, double* ll,
                                 double* grad_vr_concat);
 
+/* Model gradient: besides (ll, grad), dlogl_dP [N-1][R][S][S] = d logL /
+ * d P_{v,r}[s][s1] for every branch and category (row s = parent state, the
+ * layout of P) and dlogl_dpi [S] = d logL / d pi_s through the root term
+ * log sum_s pi_s Ft_s (VectorizedFunctions.jl:76-87).  With the substitution
+ * model's eigensystem (P = U diag(exp(mu rho t D)) Uinv,
+ * SubstitutionModels.jl:62-74) they give the gradient with respect to the
+ * model's own parameters (GTR exchangeabilities and base frequencies) by the
+ * chain rule on the host.  Per item, concatenated for a batch.  Needs S <= 8
+ * (MCP_ERR_UNSUPPORTED otherwise); runs every pre-order level as its own
+ * launch whatever the schedule.  No reference counterpart: the reference's
+ * gradient covers branch lengths only (FelsensteinFunction/6,
+ * LikelihoodCalculator_Node.jl). */
+int mcp_logl_grad_model(mcp_plan* plan, const double* blv, double* ll, double* grad,
+                        double* dlogl_dP, double* dlogl_dpi);
+int mcp_batch_logl_grad_model(mcp_batch* batch, const double* blv_concat, double* ll,
+                              double* grad_concat, double* dlogl_dP_concat,
+                              double* dlogl_dpi_concat);
+
 /* ---- device leapfrog (NUTS / HMC over branch lengths) ------------------- */
 
 /* Branch-length prior added to the log posterior of item `item` by the

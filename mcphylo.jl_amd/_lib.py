@@ -46,6 +46,7 @@ EXPORTED = (
     "mcp_mlp_param_grad", "mcp_mlp_predict", "mcp_mlp_leapfrog", "mcp_mlp_use_graph",
     "mcp_mlp_train_graph",
     "mcp_logl_grad_by_rate", "mcp_batch_logl_grad_by_rate",
+    "mcp_logl_grad_model", "mcp_batch_logl_grad_model",
     "mcp_nuts_create", "mcp_nuts_destroy", "mcp_nuts_use_batch", "mcp_nuts_use_mlp",
     "mcp_nuts_use_callback", "mcp_nuts_run", "mcp_nuts_get", "mcp_nuts_set",
     "mcp_last_error",
@@ -183,6 +184,8 @@ def load_library():
         "mcp_mlp_train_graph": (ctypes.c_int, [vp, ctypes.c_int32]),
         "mcp_logl_grad_by_rate": (ctypes.c_int, [vp, _dp, _dp, _dp]),
         "mcp_batch_logl_grad_by_rate": (ctypes.c_int, [vp, _dp, _dp, _dp]),
+        "mcp_logl_grad_model": (ctypes.c_int, [vp, _dp, _dp, _dp, _dp, _dp]),
+        "mcp_batch_logl_grad_model": (ctypes.c_int, [vp, _dp, _dp, _dp, _dp, _dp]),
         "mcp_nuts_create": (ctypes.c_int, [pp, ctypes.c_int32, _ip, _dp,
                                            ctypes.POINTER(ctypes.c_uint64), ctypes.c_double,
                                            ctypes.c_int32, ctypes.c_int32]),

@@ -37,6 +37,9 @@ int max_tile_cols(int S);
 int tile_cols(int cpt);
 hipError_t launch_reduce(const RedOp*, int, const double*, double*, double*, hipStream_t);
 bool supported_states(int S);
+bool mg_available(int S, int cpt);
+hipError_t launch_level_mg(int, int, const LevelJob*, const int32_t*, const int32_t*, int, int, int,
+                           double*, double*, hipStream_t);
 hipError_t launch_lf_pre(const HmcItem*, int, double*, double*, const double*, const double*,
                          double*, int, hipStream_t);
 hipError_t launch_lf_post(const HmcItem*, int, const double*, double*, double*, const double*,
@@ -184,6 +187,7 @@ struct ItemHost {
     int parts_off = 0;
     int out_off = 0;
     int ntiles = 0;
+    long long mg_off = -1;  // doubles into the model-gradient partial sums, -1 if not laid out
     int children_off = 0;  // int32 into children arena
     int nodes_off = 0;     // NodeInfo into nodes arena
     int leafof_off = 0;    // int32 into leaf_of arena
@@ -242,6 +246,18 @@ struct mcp_engine {
     size_t total_vr = 0;
     DevBuf dred_vr, dout_vr;
     PinnedBuf hout_vr;
+    // model gradient (d logL / dP per branch and category, d logL / d pi):
+    // every pre-order level as k_level_mg launches (whatever the schedule),
+    // partial sums in dmparts (allocated on first use), one sum over the tiles
+    // per output into dout_mg (layout per item: [branch][r][S*S], then [S])
+    std::vector<Launch> adj_mg;
+    bool mg_ok = false;  // every item has model-gradient kernels and 32-bit offsets fit
+    long long total_mparts = 0;
+    std::vector<RedOp> red_mg;
+    std::vector<size_t> mg_out_off;
+    size_t total_mg = 0;
+    DevBuf dmparts, dred_mg, dout_mg;
+    PinnedBuf hout_mg;
     long long nelemP = 0, nelemT = 0;
     size_t total_blv = 0, total_out = 0;
     long long total_patterns = 0;
@@ -647,7 +663,8 @@ static int rebuild_schedule(mcp_engine* e) {
             c.G = gb + (size_t)n.gslot * slab;
         }
         c.gpart = it.parts_off + n.gpart;
-        c.pad_ = 0;
+        c.mpart = it.mg_off < 0 ? -1
+                                : (int32_t)(it.mg_off + (long long)u * it.R * it.S * it.S * it.ntiles);
         return c;
     };
     auto make_job = [&](int i, int v) {
@@ -675,6 +692,10 @@ static int rebuild_schedule(mcp_engine* e) {
         }
         J.weights = e->dweights.as<double>() + d.weights_off;
         J.pi = e->dmodel.as<double>() + it.model_off + 2 * it.S * it.S + it.S + it.R;
+        J.tiptab = e->dtiptab.as<double>() + d.tiptab_off;
+        J.rpart = (n.is_root && it.mg_off >= 0)
+                      ? (int32_t)(it.mg_off + (long long)(it.N - 1) * it.R * it.S * it.S * it.ntiles)
+                      : -1;
         J.more = nullptr;
         for (int q = 0; q < n.nchild; ++q) {
             const ChildRef c = child_ref(it, d, it.children[n.first_child + q]);
@@ -760,7 +781,8 @@ static int rebuild_schedule(mcp_engine* e) {
         }
         return b;
     };
-    auto add_launch = [&](bool adjoint, int S, const std::vector<std::pair<int, int>>& nodes) -> int {
+    auto add_launch = [&](bool adjoint, int S, const std::vector<std::pair<int, int>>& nodes,
+                          std::vector<Launch>* target = nullptr) -> int {
         std::vector<LevelJob>& js = adjoint ? ja : jf;
         Launch L;
         L.adjoint = adjoint;
@@ -799,7 +821,7 @@ static int rebuild_schedule(mcp_engine* e) {
             while (off[j + 1] <= t0) ++j;
             e->blkjob.push_back(j);
         }
-        (adjoint ? e->adj : e->fwd).push_back(L);
+        (target ? *target : (adjoint ? e->adj : e->fwd)).push_back(L);
         return MCP_OK;
     };
     for (int h = 1; h <= maxh; ++h)
@@ -808,6 +830,20 @@ static int rebuild_schedule(mcp_engine* e) {
     for (int dd = 0; dd <= maxd; ++dd)
         for (int S : svals)
             if ((st = add_launch(true, S, by_d[dd]))) return st;
+    // model-gradient pass: every pre-order level, one k_level_mg launch per
+    // (depth, state count)
+    e->adj_mg.clear();
+    if (e->mg_ok) {
+        std::vector<std::vector<std::pair<int, int>>> all_d(maxd + 1);
+        for (int i = 0; i < nitems; ++i) {
+            const ItemHost& it = e->items[i];
+            for (int v = 0; v < it.N; ++v)
+                if (it.nodes[v].nchild > 0) all_d[it.depth[v]].push_back({i, v});
+        }
+        for (int dd = 0; dd <= maxd; ++dd)
+            for (int S : svals)
+                if ((st = add_launch(true, S, all_d[dd], &e->adj_mg))) return st;
+    }
     if (e->schedule != MCP_SCHEDULE_LEVELS) {
         // fused: jobs grouped by item, then level; per-item level offsets
         std::vector<int32_t> lf, la;
@@ -929,8 +965,52 @@ static int rebuild_schedule(mcp_engine* e) {
     if ((st = upload(e, e->dred_vr, e->red_vr.data(), e->red_vr.size() * sizeof(RedOp)))) return st;
     if ((st = e->dout_vr.ensure(std::max<size_t>(e->total_vr, 1) * sizeof(double)))) return st;
     if ((st = e->hout_vr.ensure(std::max<size_t>(e->total_vr, 1) * sizeof(double)))) return st;
+    e->red_mg.clear();
+    e->mg_out_off.assign(e->items.size(), 0);
+    e->total_mg = 0;
+    if (e->mg_ok) {
+        for (size_t i = 0; i < e->items.size(); ++i) {
+            const ItemHost& it = e->items[i];
+            const int SS = it.S * it.S;
+            e->mg_out_off[i] = e->total_mg;
+            // per (branch, r, e): the tiles of mparts[mg_off + ((v R + r) SS + e) ntiles ..]
+            for (long long q = 0; q < (long long)(it.N - 1) * it.R * SS; ++q)
+                e->red_mg.push_back(RedOp{(int32_t)(e->total_mg + q),
+                                          (int32_t)(it.mg_off + q * it.ntiles), it.ntiles, 1});
+            const size_t root = e->total_mg + (size_t)(it.N - 1) * it.R * SS;
+            const long long rbase = it.mg_off + (long long)(it.N - 1) * it.R * SS * it.ntiles;
+            // per state: the (r, tile) block of the root term
+            for (int s2 = 0; s2 < it.S; ++s2)
+                e->red_mg.push_back(RedOp{(int32_t)(root + s2),
+                                          (int32_t)(rbase + (long long)s2 * it.R * it.ntiles),
+                                          it.R * it.ntiles, 1});
+            e->total_mg += (size_t)(it.N - 1) * it.R * SS + it.S;
+        }
+        if ((st = upload(e, e->dred_mg, e->red_mg.data(), e->red_mg.size() * sizeof(RedOp))))
+            return st;
+        if ((st = e->dout_mg.ensure(std::max<size_t>(e->total_mg, 1) * sizeof(double)))) return st;
+        if ((st = e->hout_mg.ensure(std::max<size_t>(e->total_mg, 1) * sizeof(double)))) return st;
+    }
     if ((st = upload(e, e->dred, e->red.data(), e->red.size() * sizeof(RedOp)))) return st;
     return MCP_OK;
+}
+
+// Offsets of the model-gradient partial sums (per item: [branch][r][S*S][tile]
+// then the root's [S][r][tile]); the pass exists only when every item has
+// model-gradient kernels for its tile width and the offsets fit 32 bits.
+static void assign_mg_offsets(mcp_engine* e) {
+    long long tot = 0;
+    bool ok = true;
+    for (auto& it : e->items) {
+        if (!mg_available(it.S, e->cpt(it.S))) ok = false;
+        it.mg_off = tot;
+        tot += ((long long)(it.N - 1) * it.R * it.S * it.S + (long long)it.S * it.R) * it.ntiles;
+    }
+    if (tot > 0x7fffffffLL) ok = false;
+    e->mg_ok = ok;
+    e->total_mparts = ok ? tot : 0;
+    if (!ok)
+        for (auto& it : e->items) it.mg_off = -1;
 }
 
 static int engine_create(mcp_engine** out, const mcp_batch_desc* d) {
@@ -1021,6 +1101,7 @@ static int engine_create(mcp_engine** out, const mcp_batch_desc* d) {
         if ((st = upload(e.get(), e->dtiptab, ht.data(), ht.size() * sizeof(double)))) return st;
         if ((st = upload(e.get(), e->dweights, hw.data(), hw.size() * sizeof(double)))) return st;
     }
+    assign_mg_offsets(e.get());
     if ((st = rebuild_schedule(e.get()))) return st;
     *out = e.release();
     return MCP_OK;
@@ -1044,6 +1125,7 @@ static int relayout_tiles(mcp_engine* e) {
     int st;
     if ((st = e->dparts.ensure((size_t)std::max<long long>(parts, 1) * sizeof(double)))) return st;
     HIP_TRY(hipMemset(e->dparts.p, 0, (size_t)std::max<long long>(parts, 1) * sizeof(double)));
+    assign_mg_offsets(e);
     return rebuild_schedule(e);
 }
 
@@ -1268,6 +1350,76 @@ static int engine_eval_by_rate(mcp_engine* e, const double* blv, double* ll, dou
     const double* o = e->hout.as<double>();
     for (size_t i = 0; i < e->items.size(); ++i) ll[i] = o[e->items[i].out_off];
     std::memcpy(grad_vr, e->hout_vr.p, e->total_vr * sizeof(double));
+    return MCP_OK;
+}
+
+// logL, branch-length gradient and the model gradient (eager launches):
+// P build, the post-order pass of the current schedule (its fused launches
+// without their pre-order levels), every pre-order level with k_level_mg,
+// the reductions.  dP[item] is [branch][r][S][S] (P[s][s1] layout, row =
+// parent state), dpi[item] is [S] (root term only).
+static int engine_eval_model(mcp_engine* e, const double* blv, double* ll, double* grad,
+                             double* dP, double* dpi) {
+    if (!e || !blv || !ll || !grad || !dP || !dpi) return fail(MCP_ERR_INVALID, "null argument");
+    for (auto& it : e->items)
+        if (!it.model_set) return fail(MCP_ERR_STATE, "substitution model not set");
+    if (!e->mg_ok)
+        return fail(MCP_ERR_UNSUPPORTED,
+                    "model gradient needs S <= 8 with a built tile width for every item");
+    DeviceGuard guard(e->device);
+    int st;
+    if ((st = e->dmparts.ensure((size_t)std::max<long long>(e->total_mparts, 1) * sizeof(double))))
+        return st;
+    std::memcpy(e->hblv.p, blv, e->total_blv * sizeof(double));
+    e->evaluated = true;
+    hipStream_t s = e->stream;
+    HIP_TRY(hipMemcpyAsync(e->dblv.p, e->hblv.p, e->total_blv * sizeof(double),
+                           hipMemcpyHostToDevice, s));
+    HIP_TRY(launch_build_P(e->dmodels.as<ModelDev>(), e->delemP.as<int32_t>(),
+                           (int)e->items.size(), e->nelemP, s));
+    HIP_TRY(launch_tip_tables(e->dmodels.as<ModelDev>(), e->delemT.as<int32_t>(),
+                              (int)e->items.size(), e->nelemT, s));
+    for (const Launch& L : e->fwd)
+        HIP_TRY(launch_level(false, L.S, L.cpt, e->djobs_f.as<LevelJob>() + L.job_begin,
+                             e->dtaskoff.as<int32_t>() + L.taskoff_begin,
+                             e->dblkjob.as<int32_t>() + L.blk_begin, L.ntasks, L.chunk, L.grid,
+                             e->dparts.as<double>(), s));
+    for (const auto& F : e->fused)
+        HIP_TRY(launch_fused(false, F.S, F.cpt, e->dsched.as<ItemSched>() + F.sched_begin,
+                             F.nitems, F.nblocks, e->waves, e->djobs_f.as<LevelJob>(),
+                             e->dlvl_f.as<int32_t>(), e->djobs_a.as<LevelJob>(),
+                             e->dlvl_a.as<int32_t>(), e->dparts.as<double>(), s));
+    for (const Launch& L : e->adj_mg)
+        HIP_TRY(launch_level_mg(L.S, L.cpt, e->djobs_a.as<LevelJob>() + L.job_begin,
+                                e->dtaskoff.as<int32_t>() + L.taskoff_begin,
+                                e->dblkjob.as<int32_t>() + L.blk_begin, L.ntasks, L.chunk, L.grid,
+                                e->dparts.as<double>(), e->dmparts.as<double>(), s));
+    HIP_TRY(launch_reduce(e->dred.as<RedOp>(), (int)e->red.size(), e->dparts.as<double>(),
+                          e->dout.as<double>(), e->dredtmp.as<double>(), s));
+    if (!e->red2.empty())
+        HIP_TRY(launch_reduce(e->dred2.as<RedOp>(), (int)e->red2.size(), e->dredtmp.as<double>(),
+                              e->dout.as<double>(), e->dredtmp.as<double>(), s));
+    HIP_TRY(launch_reduce(e->dred_mg.as<RedOp>(), (int)e->red_mg.size(), e->dmparts.as<double>(),
+                          e->dout.as<double>(), e->dout_mg.as<double>(), s));
+    HIP_TRY(hipMemcpyAsync(e->hout.p, e->dout.p, e->total_out * sizeof(double),
+                           hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(e->hout_mg.p, e->dout_mg.p, e->total_mg * sizeof(double),
+                           hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    const double* o = e->hout.as<double>();
+    const double* om = e->hout_mg.as<double>();
+    size_t goff = 0, poff = 0, soff = 0;
+    for (size_t i = 0; i < e->items.size(); ++i) {
+        const ItemHost& it = e->items[i];
+        ll[i] = o[it.out_off];
+        std::memcpy(grad + goff, o + it.out_off + 1, (it.N - 1) * sizeof(double));
+        goff += it.N - 1;
+        const size_t nP = (size_t)(it.N - 1) * it.R * it.S * it.S;
+        std::memcpy(dP + poff, om + e->mg_out_off[i], nP * sizeof(double));
+        poff += nP;
+        std::memcpy(dpi + soff, om + e->mg_out_off[i] + nP, it.S * sizeof(double));
+        soff += it.S;
+    }
     return MCP_OK;
 }
 
@@ -1517,6 +1669,17 @@ int mcp_hmc_set_prior(mcp_batch* b, int32_t item, int32_t kind, double scale) {
 
 int mcp_logl_grad_by_rate(mcp_plan* p, const double* blv, double* ll, double* grad_vr) {
     return engine_eval_by_rate(p, blv, ll, grad_vr);
+}
+
+int mcp_logl_grad_model(mcp_plan* p, const double* blv, double* ll, double* grad, double* dlogl_dP,
+                        double* dlogl_dpi) {
+    return engine_eval_model(p, blv, ll, grad, dlogl_dP, dlogl_dpi);
+}
+
+int mcp_batch_logl_grad_model(mcp_batch* b, const double* blv_concat, double* ll,
+                              double* grad_concat, double* dlogl_dP_concat,
+                              double* dlogl_dpi_concat) {
+    return engine_eval_model(b, blv_concat, ll, grad_concat, dlogl_dP_concat, dlogl_dpi_concat);
 }
 
 int mcp_batch_logl_grad_by_rate(mcp_batch* b, const double* blv_concat, double* ll,

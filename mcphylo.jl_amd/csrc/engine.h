@@ -26,7 +26,8 @@ struct ChildRef {
     const double* dP;      // same layouts for dP/dt (pre-order pass)
     double* G;             // internal child: its pre-order slab at r = 0; leaf: nullptr
     int32_t gpart;         // offset of the branch's gradient partial sums
-    int32_t pad_;
+    int32_t mpart;         // offset of the branch's d logL / dP partial sums (model-gradient
+                           // buffer, layout [r][S*S][tile]), -1 when not laid out
 };
 
 constexpr int kInlineChildren = 3;
@@ -39,11 +40,13 @@ struct LevelJob {
     int32_t nchild;
     int32_t lpart;          // offset of this node's log-scaler partial sums
     int32_t is_root;
-    int32_t pad_;
+    int32_t rpart;          // root: offset of the d logL / d pi partial sums (model-gradient
+                            // buffer, layout [S][r][tile]), else -1
     double* F;              // output partial slab at r = 0 (non-root), else nullptr
     const double* G;        // own pre-order slab at r = 0 (non-root), else nullptr (root: pi)
     const double* weights;  // [Cp], zero in the padding
     const double* pi;       // [S]
+    const double* tiptab;   // [K][S] tip vectors of the state codes
     const ChildRef* more;   // children beyond kInlineChildren
     ChildRef kid[kInlineChildren];
 };

@@ -241,7 +241,8 @@ template <int S, int CPT>
 __device__ __forceinline__ double child_adjoint(const ChildRef& ch, const LevelJob& J, int r, int c0,
                                                 const double (&T)[CPT][S],
                                                 const double (&f)[CPT][S],
-                                                const double (&m)[CPT][S]) {
+                                                const double (&m)[CPT][S],
+                                                double (&colv)[CPT]) {
     const MCP_GLOBAL double* wts = gptr(J.weights);
     double contrib = 0.0;
     if (ch.codes) {
@@ -258,6 +259,7 @@ __device__ __forceinline__ double child_adjoint(const ChildRef& ch, const LevelJ
                 num = fma(T[k][s], row[s], num);
                 col = fma(T[k][s], m[k][s], col);
             }
+            colv[k] = col;
             const double v = wts[c] * (num / col);
             contrib += c < J.C ? v : 0.0;
         }
@@ -292,6 +294,7 @@ __device__ __forceinline__ double child_adjoint(const ChildRef& ch, const LevelJ
 #pragma unroll
             for (int s = 0; s < S; ++s) g[s] *= inv;
             store_row<S>(Gdst + (size_t)c * S, g);
+            colv[k] = col;
             const double v = wts[c] * (num / col);
             contrib += c < J.C ? v : 0.0;
         }
@@ -299,9 +302,86 @@ __device__ __forceinline__ double child_adjoint(const ChildRef& ch, const LevelJ
     return contrib;
 }
 
+// Model gradient (the substitution model's parameters enter through every
+// P): d logL / d P_u[m][s] for child u in category r is
+//   A[m][s] = sum_c w_c T_u[c][m] f_u[c][s] / col_c
+// (f_u: the child's partial, or its tip vector), since d col_c = sum_{m,s}
+// T_u[m] dP[m][s] f_u[s] and d logL = sum_c w_c d col_c / col_c.  The
+// wavefront's sums go to mparts[mpart + (r S^2 + e) ntiles + tile].
 template <int S, int CPT>
+__device__ __forceinline__ void mg_branch(const ChildRef& ch, const LevelJob& J, int r, int tile,
+                                          int c0, const double (&T)[CPT][S],
+                                          const double (&f)[CPT][S], const double (&colv)[CPT],
+                                          double* __restrict__ mparts) {
+    const MCP_GLOBAL double* wts = gptr(J.weights);
+    double A[S][S];
+#pragma unroll
+    for (int m = 0; m < S; ++m)
+#pragma unroll
+        for (int s = 0; s < S; ++s) A[m][s] = 0.0;
+#pragma unroll
+    for (int k = 0; k < CPT; ++k) {
+        const int c = c0 + 64 * k;
+        const double wc = c < J.C ? wts[c] / colv[k] : 0.0;
+        double fv[S];
+        if (ch.codes) {
+            select_row<S>(gptr(J.tiptab), (int)gptr(ch.codes)[c], fv);
+        } else {
+#pragma unroll
+            for (int s = 0; s < S; ++s) fv[s] = f[k][s];
+        }
+#pragma unroll
+        for (int m = 0; m < S; ++m) {
+            const double tm = wc * T[k][m];
+#pragma unroll
+            for (int s = 0; s < S; ++s) A[m][s] = fma(tm, fv[s], A[m][s]);
+        }
+    }
+    double* dst = mparts + ch.mpart + ((size_t)r * S * S) * J.ntiles + tile;
+    const int lane = (int)(threadIdx.x & 63);
+#pragma unroll
+    for (int m = 0; m < S; ++m)
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            const double v = wave_sum(A[m][s]);
+            if (lane == 0) dst[(size_t)(m * S + s) * J.ntiles] = v;
+        }
+}
+
+// Root term of the model gradient: d logL / d pi_s = sum_c w_c Ft[c][s] /
+// sum_s' pi_s' Ft[c][s'] with Ft the product of the root's children messages
+// (VectorizedFunctions.jl:76-87); sums to mparts[rpart + (s R + r) ntiles + tile].
+template <int S, int CPT>
+__device__ __forceinline__ void mg_root(const LevelJob& J, int r, int tile, int c0,
+                                        const double (&Ft)[CPT][S],
+                                        double* __restrict__ mparts) {
+    const MCP_GLOBAL double* wts = gptr(J.weights);
+    const MCP_GLOBAL double* pi = gptr(J.pi);
+    double a[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s) a[s] = 0.0;
+#pragma unroll
+    for (int k = 0; k < CPT; ++k) {
+        const int c = c0 + 64 * k;
+        double t = 0.0;
+#pragma unroll
+        for (int s = 0; s < S; ++s) t = fma(pi[s], Ft[k][s], t);
+        const double wc = c < J.C ? wts[c] / t : 0.0;
+#pragma unroll
+        for (int s = 0; s < S; ++s) a[s] = fma(wc, Ft[k][s], a[s]);
+    }
+    const int lane = (int)(threadIdx.x & 63);
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        const double v = wave_sum(a[s]);
+        if (lane == 0) mparts[J.rpart + ((size_t)s * J.R + r) * J.ntiles + tile] = v;
+    }
+}
+
+template <int S, int CPT, bool MG = false>
 __device__ __forceinline__ void adj_task(const LevelJob& J, int r, int tile,
-                                         double* __restrict__ parts) {
+                                         double* __restrict__ parts,
+                                         double* __restrict__ mparts = nullptr) {
     const int lane = (int)(threadIdx.x & 63);
     const int c0 = tile * 64 * CPT + lane;
     double gp[CPT][S];
@@ -326,12 +406,24 @@ __device__ __forceinline__ void adj_task(const LevelJob& J, int r, int tile,
         for (int k = 0; k < CPT; ++k)
 #pragma unroll
             for (int s = 0; s < S; ++s) T[k][s] = gp[k][s] * mb[k][s];
-        const double ca = child_adjoint<S, CPT>(a, J, r, c0, T, fa, ma);
+        double colv[CPT];
+        const double ca = child_adjoint<S, CPT>(a, J, r, c0, T, fa, ma, colv);
+        if constexpr (MG) mg_branch<S, CPT>(a, J, r, tile, c0, T, fa, colv, mparts);
 #pragma unroll
         for (int k = 0; k < CPT; ++k)
 #pragma unroll
             for (int s = 0; s < S; ++s) T[k][s] = gp[k][s] * ma[k][s];
-        const double cb = child_adjoint<S, CPT>(b, J, r, c0, T, fb, mb);
+        const double cb = child_adjoint<S, CPT>(b, J, r, c0, T, fb, mb, colv);
+        if constexpr (MG) {
+            mg_branch<S, CPT>(b, J, r, tile, c0, T, fb, colv, mparts);
+            if (!J.G) {
+#pragma unroll
+                for (int k = 0; k < CPT; ++k)
+#pragma unroll
+                    for (int s = 0; s < S; ++s) T[k][s] = ma[k][s] * mb[k][s];
+                mg_root<S, CPT>(J, r, tile, c0, T, mparts);
+            }
+        }
         const double sa = wave_sum(ca);
         const double sb = wave_sum(cb);
         if (lane == 0) {
@@ -358,9 +450,29 @@ __device__ __forceinline__ void adj_task(const LevelJob& J, int r, int tile,
                 for (int s = 0; s < S; ++s) T[k][s] *= mw[k][s];
         }
         child_fetch<S, CPT>(u, J, r, c0, fu, mu_);
-        const double cu = child_adjoint<S, CPT>(u, J, r, c0, T, fu, mu_);
+        double colv[CPT];
+        const double cu = child_adjoint<S, CPT>(u, J, r, c0, T, fu, mu_, colv);
+        if constexpr (MG) mg_branch<S, CPT>(u, J, r, tile, c0, T, fu, colv, mparts);
         const double su = wave_sum(cu);
         if (lane == 0) parts[u.gpart + r * J.ntiles + tile] = su;
+    }
+    if constexpr (MG) {
+        if (!J.G) {
+            double Ft[CPT][S];
+#pragma unroll
+            for (int k = 0; k < CPT; ++k)
+#pragma unroll
+                for (int s = 0; s < S; ++s) Ft[k][s] = 1.0;
+            for (int q = 0; q < J.nchild; ++q) {
+                double fw[CPT][S], mw[CPT][S];
+                child_fetch<S, CPT>(child_of(J, q), J, r, c0, fw, mw);
+#pragma unroll
+                for (int k = 0; k < CPT; ++k)
+#pragma unroll
+                    for (int s = 0; s < S; ++s) Ft[k][s] *= mw[k][s];
+            }
+            mg_root<S, CPT>(J, r, tile, c0, Ft, mparts);
+        }
     }
 }
 
@@ -398,6 +510,33 @@ __attribute__((amdgpu_waves_per_eu(ADJ ? MCP_ADJ_WAVES_PER_EU : MCP_FWD_WAVES_PE
         const int tile = local - r * J.ntiles;
         if constexpr (ADJ) adj_task<S, CPT>(J, r, tile, parts);
         else fwd_task<S, CPT>(J, r, tile, parts);
+    }
+}
+
+// Pre-order level with the model gradient (d logL / dP per branch and
+// category, d logL / d pi at the root) accumulated beside the branch-length
+// gradient; same task decomposition as k_level.
+template <int S, int CPT>
+__global__ void __launch_bounds__(64) k_level_mg(const LevelJob* __restrict__ jobs,
+                                                 const int32_t* __restrict__ task_off,
+                                                 const int32_t* __restrict__ blk_job, int total,
+                                                 int chunk, double* __restrict__ parts,
+                                                 double* __restrict__ mparts) {
+    const int t0 = blockIdx.x * chunk;
+    const int t1 = min(total, t0 + chunk);
+    int j = blk_job[blockIdx.x];
+    int jbeg = task_off[j], jend = task_off[j + 1];
+    for (int t = t0; t < t1; ++t) {
+        while (t >= jend) {
+            ++j;
+            jbeg = jend;
+            jend = task_off[j + 1];
+        }
+        const LevelJob& J = jobs[j];
+        const int local = t - jbeg;
+        const int r = local / J.ntiles;
+        const int tile = local - r * J.ntiles;
+        adj_task<S, CPT, true>(J, r, tile, parts, mparts);
     }
 }
 
@@ -511,6 +650,12 @@ hipError_t launch_tip_tables(const ModelDev* models, const int32_t* elem_off, in
     X(1, 2) X(2, 1) X(2, 2) X(2, 4) X(3, 1) X(3, 2) X(3, 4) X(4, 1) X(4, 2) X(4, 4) X(5, 1) \
     X(6, 1) X(7, 1) X(8, 1) X(16, 1) X(20, 1)
 
+// The (S, CPT) pairs with model-gradient kernels (S <= 8: the per-lane
+// S x S accumulator stays in registers).
+#define MCP_FOR_EACH_SC_MG(X)                                                                \
+    X(1, 2) X(2, 1) X(2, 2) X(2, 4) X(3, 1) X(3, 2) X(3, 4) X(4, 1) X(4, 2) X(4, 4) X(5, 1) \
+    X(6, 1) X(7, 1) X(8, 1)
+
 #ifndef MCP_MFMA20
 #define MCP_MFMA20 1
 #endif
@@ -555,6 +700,28 @@ int max_tile_cols(int S) {
 }
 
 int tile_cols(int cpt) { return 64 * (cpt < 1 ? 1 : cpt); }
+
+bool mg_available(int S, int cpt) {
+#define X(n, c) if (S == n && cpt == c) return true;
+    MCP_FOR_EACH_SC_MG(X)
+#undef X
+    return false;
+}
+
+hipError_t launch_level_mg(int S, int cpt, const LevelJob* jobs, const int32_t* task_off,
+                           const int32_t* blk_job, int total, int chunk, int grid, double* parts,
+                           double* mparts, hipStream_t st) {
+    if (total <= 0 || grid <= 0) return hipSuccess;
+#define X(n, c)                                                                                  \
+    if (S == n && cpt == c) {                                                                    \
+        hipLaunchKernelGGL((k_level_mg<n, c>), dim3(grid), dim3(64), 0, st, jobs, task_off,      \
+                           blk_job, total, chunk, parts, mparts);                                \
+        return hipGetLastError();                                                                \
+    }
+    MCP_FOR_EACH_SC_MG(X)
+#undef X
+    return hipErrorInvalidValue;
+}
 
 hipError_t launch_level20(bool adjoint, const LevelJob* jobs, const int32_t* task_off,
                           const int32_t* blk_job, int total, int chunk, int grid, double* parts,

@@ -260,6 +260,21 @@ class Plan(_Handle):
         b = np.asarray(blv, dtype=np.float64)
         return ll, (b @ G) / np.asarray(rates, dtype=np.float64)
 
+    def logl_grad_model(self, blv):
+        """(ll, grad [N-1], dP [N-1, R, S, S], dpi [S]): besides the branch
+        gradient, d ll / d P of every branch and category (row = parent
+        state) and d ll / d pi through the root term (S <= 8)."""
+        b = np.ascontiguousarray(blv, dtype=np.float64)
+        if b.shape != (self.N - 1,):
+            raise _lib.McpDimensionMismatch(_lib.MCP_ERR_DIMENSION, "blv must have length N-1")
+        out = np.zeros(1)
+        g = np.zeros(self.N - 1)
+        dP = np.zeros((self.N - 1, self.R, self.S, self.S))
+        dpi = np.zeros(self.S)
+        check(_lib.load_library().mcp_logl_grad_model(self._h, dptr(b), dptr(out), dptr(g),
+                                                      dptr(dP), dptr(dpi)))
+        return float(out[0]), g, dP, dpi
+
 
 class Batch(_Handle):
     """Ragged batch of items.
@@ -366,6 +381,25 @@ class Batch(_Handle):
         off = np.concatenate([[0], np.cumsum(sizes)])
         return ll, [g[off[i]:off[i + 1]].reshape(self.item_nodes[i] - 1, self.item_R[i])
                     for i in range(self.n_items)]
+
+    def logl_grad_model(self, blv_concat):
+        """(ll [items], grad_concat, [dP_i [N_i-1, R_i, S_i, S_i]], [dpi_i [S_i]]):
+        the model gradient of every item (see Plan.logl_grad_model)."""
+        b = self._blv(blv_concat)
+        ll = np.zeros(self.n_items)
+        g = np.zeros(b.shape[0])
+        sizes = [(n - 1) * r * s * s
+                 for n, r, s in zip(self.item_nodes, self.item_R, self.item_S)]
+        dP = np.zeros(int(np.sum(sizes)))
+        dpi = np.zeros(int(np.sum(self.item_S)))
+        check(_lib.load_library().mcp_batch_logl_grad_model(self._h, dptr(b), dptr(ll), dptr(g),
+                                                            dptr(dP), dptr(dpi)))
+        off = np.concatenate([[0], np.cumsum(sizes)])
+        soff = np.concatenate([[0], np.cumsum(self.item_S)])
+        dPs = [dP[off[i]:off[i + 1]].reshape(self.item_nodes[i] - 1, self.item_R[i],
+                                             self.item_S[i], self.item_S[i])
+               for i in range(self.n_items)]
+        return ll, g, dPs, [dpi[soff[i]:soff[i + 1]] for i in range(self.n_items)]
 
     def split(self, grad_concat):
         return [grad_concat[self.blv_off[i]:self.blv_off[i + 1]] for i in range(self.n_items)]

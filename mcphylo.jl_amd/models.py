@@ -16,7 +16,8 @@ import math
 
 import numpy as np
 
-__all__ = ["Restriction", "JC", "GTR", "freeK", "setmatrix", "ComplexEigensystemError"]
+__all__ = ["Restriction", "JC", "GTR", "freeK", "setmatrix", "ComplexEigensystemError",
+           "dlogl_dM", "gtr_gradient", "restriction_gradient"]
 
 
 class ComplexEigensystemError(ValueError):
@@ -127,3 +128,84 @@ def stationary_freeK(substitution_rates) -> np.ndarray:
     E = np.zeros_like(D)
     E[-1] = 1.0
     return np.real((U @ np.diag(E) @ Uinv)[0, :])
+
+
+# ---------------------------------------------------------------------------
+# Gradient with respect to the substitution model's parameters
+# ---------------------------------------------------------------------------
+# The device returns d logL / d P_{v,r} (every branch v and category r) and
+# d logL / d pi through the root term.  P_{v,r} = exp(tau M) with tau = rho_r
+# t_v and M = mu Q = U diag(mu D) Uinv, so by the Daleckii-Krein formula
+#   dP = U [(Uinv dM U) o Phi] Uinv,  Phi_ij = tau (e^{a_i} - e^{a_j}) / (a_i - a_j),
+# a = tau mu D (Phi_ii = tau e^{a_i}), and
+#   d logL / dM = Uinv^T [sum_{v,r} (U^T A_{v,r} Uinv^T) o Phi_{v,r}] U^T.
+# The model's parameters then enter through M = mu(theta) Q(theta).
+
+
+def _exp_divided_difference(a: np.ndarray) -> np.ndarray:
+    """(e^{a_i} - e^{a_j}) / (a_i - a_j) over the last axis pairs (e^{a_i}
+    on the diagonal and for equal values), evaluated as e^{a_j} expm1(d) / d
+    with d = a_i - a_j, so nearly equal eigenvalues lose no digits."""
+    ai = a[..., :, None]
+    aj = a[..., None, :]
+    d = ai - aj
+    with np.errstate(invalid="ignore", divide="ignore"):
+        q = np.where(d == 0.0, 1.0, np.expm1(d) / np.where(d == 0.0, 1.0, d))
+    return np.exp(aj) * q
+
+
+def dlogl_dM(U, D, Uinv, mu, blv, rates, dP) -> np.ndarray:
+    """d logL / dM (S x S) for M = mu Q, from dP [N-1, R, S, S] = d logL /
+    d P_{v,r} and the eigensystem the likelihood was evaluated with."""
+    U = np.asarray(U, dtype=np.float64)
+    Ui = np.asarray(Uinv, dtype=np.float64)
+    D = np.asarray(D, dtype=np.float64)
+    tau = np.asarray(blv, dtype=np.float64)[:, None] * np.asarray(rates, dtype=np.float64)[None, :]
+    a = tau[..., None] * (float(mu) * D)[None, None, :]
+    Phi = tau[..., None, None] * _exp_divided_difference(a)
+    X = U.T[None, None] @ np.asarray(dP, dtype=np.float64) @ Ui.T[None, None]
+    return Ui.T @ np.sum(X * Phi, axis=(0, 1)) @ U.T
+
+
+def gtr_gradient(base_freq, substitution_rates, U, D, Uinv, mu, blv, rates, dP, dpi):
+    """(d logL / d substitution_rates [S(S-1)/2, setmatrix order],
+    d logL / d base_freq [S]) of the GTR model (`SubstitutionModels.jl:62-74`):
+    Q[i,j] = R[i,j] pi[j] (i != j), Q[i,i] = -sum_j Q[i,j], mu = 1 / sum of the
+    row sums; pi also weights the root term.  The frequencies are taken as
+    given (no renormalisation), as the reference's GTR takes them."""
+    pi = np.asarray(base_freq, dtype=np.float64)
+    S = pi.shape[0]
+    Rm = setmatrix(substitution_rates)
+    Q = Rm * pi[None, :]
+    Q[np.diag_indices(S)] = -Q.sum(axis=1)
+    G = dlogl_dM(U, D, Uinv, mu, blv, rates, dP)
+    gQ = float(np.sum(G * Q))          # <G, Q>: the mu channel
+    mu = float(mu)
+    d_exch = []
+    for j in range(S):
+        for i in range(j):
+            dq = G[i, j] * pi[j] + G[j, i] * pi[i] - G[i, i] * pi[j] - G[j, j] * pi[i]
+            dmu = -mu * mu * (pi[i] + pi[j])
+            d_exch.append(dmu * gQ + mu * dq)
+    d_pi = np.zeros(S)
+    for j in range(S):
+        dq = sum(Rm[i, j] * (G[i, j] - G[i, i]) for i in range(S) if i != j)
+        dmu = -mu * mu * sum(Rm[i, j] for i in range(S) if i != j)
+        d_pi[j] = dmu * gQ + mu * dq + float(np.asarray(dpi)[j])
+    return np.array(d_exch), d_pi
+
+
+def restriction_gradient(base_freq, U, D, Uinv, mu, blv, rates, dP, dpi):
+    """d logL / d base_freq [2] of the restriction model
+    (`SubstitutionModels.jl:13-24`): its U (hence Q = [[-p2, p2], [1-p2,
+    -(1-p2)]]) depends on pi[2] only, mu = 1 / (2 pi[1] pi[2]) on both, and pi
+    weights the root term."""
+    f = np.asarray(base_freq, dtype=np.float64)
+    Q = np.array([[-f[1], f[1]], [1.0 - f[1], -(1.0 - f[1])]])
+    G = dlogl_dM(U, D, Uinv, mu, blv, rates, dP)
+    gQ = float(np.sum(G * Q))
+    mu = float(mu)
+    dQ2 = np.array([[-1.0, 1.0], [-1.0, 1.0]])
+    d1 = (-mu / f[0]) * gQ + float(dpi[0])
+    d2 = (-mu / f[1]) * gQ + mu * float(np.sum(G * dQ2)) + float(dpi[1])
+    return np.array([d1, d2])
