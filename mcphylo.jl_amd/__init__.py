@@ -21,7 +21,7 @@ from .engine import Batch, Model, Plan
 from .models import GTR, JC, ComplexEigensystemError, Restriction, freeK, setmatrix
 from .nexus import ParseNexus, compress_patterns, datafortree
 from .phylodist import (MultiplePhyloDist, PhyloDist, __logpdf, clear_plan_cache, gradlogpdf,
-                        logpdf, maximum, minimum, size)
+                        gradlogpdf_params, logpdf, maximum, minimum, size)
 from .rates import discrete_gamma_rates, mean_boundaries, median_boundaries
 from . import priors, sampler, surrogate, synthetic
 from .surrogate import Surrogate
@@ -34,7 +34,8 @@ __all__ = [
     "Plan", "Batch", "Model", "Surrogate",
     "JC", "GTR", "Restriction", "freeK", "setmatrix", "ComplexEigensystemError",
     "ParseNexus", "datafortree", "compress_patterns",
-    "PhyloDist", "MultiplePhyloDist", "logpdf", "gradlogpdf", "__logpdf", "clear_plan_cache",
+    "PhyloDist", "MultiplePhyloDist", "logpdf", "gradlogpdf", "gradlogpdf_params", "__logpdf",
+    "clear_plan_cache",
     "size", "minimum", "maximum",
     "discrete_gamma_rates", "mean_boundaries", "median_boundaries",
     "Node", "ParseNewick", "TreeArrays", "tree_arrays", "tree_from_parent", "post_order",

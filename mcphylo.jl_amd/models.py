@@ -17,7 +17,7 @@ import math
 import numpy as np
 
 __all__ = ["Restriction", "JC", "GTR", "freeK", "setmatrix", "ComplexEigensystemError",
-           "dlogl_dM", "gtr_gradient", "restriction_gradient"]
+           "dlogl_dM", "gtr_gradient", "restriction_gradient", "rate_gradient"]
 
 
 class ComplexEigensystemError(ValueError):
@@ -209,3 +209,17 @@ def restriction_gradient(base_freq, U, D, Uinv, mu, blv, rates, dP, dpi):
     d1 = (-mu / f[0]) * gQ + float(dpi[0])
     d2 = (-mu / f[1]) * gQ + mu * float(np.sum(G * dQ2)) + float(dpi[1])
     return np.array([d1, d2])
+
+
+def rate_gradient(U, D, Uinv, mu, blv, rates, dP) -> np.ndarray:
+    """d logL / d rho_r [R] of the category rates from dP [N-1, R, S, S]:
+    sum_v <A_{v,r}, dP_{v,r}/d rho_r>, dP/d rho = U diag(mu t D e^{a}) Uinv."""
+    U = np.asarray(U, dtype=np.float64)
+    Ui = np.asarray(Uinv, dtype=np.float64)
+    D = np.asarray(D, dtype=np.float64)
+    t = np.asarray(blv, dtype=np.float64)
+    rho = np.asarray(rates, dtype=np.float64)
+    a = (t[:, None] * rho[None, :])[..., None] * (float(mu) * D)[None, None, :]
+    scale = (float(mu) * t[:, None, None] * D[None, None, :]) * np.exp(a)
+    dPdrho = (U[None, None] * scale[..., None, :]) @ Ui[None, None]
+    return np.sum(np.asarray(dP, dtype=np.float64) * dPdrho, axis=(0, 2, 3))

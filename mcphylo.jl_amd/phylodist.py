@@ -26,11 +26,12 @@ import numpy as np
 
 from . import _lib
 from .engine import Batch, Model, Plan
-from .models import freeK, stationary_freeK
+from .models import (GTR, JC, Restriction, freeK, gtr_gradient, rate_gradient,
+                     restriction_gradient, stationary_freeK)
 from .tree import Node, get_branchlength_vector, post_order, tree_arrays
 
-__all__ = ["PhyloDist", "MultiplePhyloDist", "logpdf", "gradlogpdf", "__logpdf", "logpdf_multi",
-           "clear_plan_cache", "size", "minimum", "maximum"]
+__all__ = ["PhyloDist", "MultiplePhyloDist", "logpdf", "gradlogpdf", "gradlogpdf_params",
+           "__logpdf", "logpdf_multi", "clear_plan_cache", "size", "minimum", "maximum"]
 
 
 class PhyloDist:
@@ -181,6 +182,33 @@ def gradlogpdf(d: PhyloDist, x):
     (`Phylodist.jl:124-138`)."""
     plan = _plan_for(d, x)
     return plan.logl_grad(get_branchlength_vector(d.tree))
+
+
+def gradlogpdf_params(d: PhyloDist, x):
+    """(ll, {name: gradient}) with respect to the branch lengths and the
+    model's own parameters: 'branch_lengths' [N-1] by num, 'rates' [R]
+    (category rates), 'base_freq' [S] and 'substitution_rates' (GTR
+    exchangeabilities in setmatrix order; zeros for models that ignore them).
+    GTR, Restriction and JC with S <= 8.  No reference counterpart: the
+    reference differentiates the branch lengths only (`Phylodist.jl:124-138`)."""
+    if d.substitution_model not in (GTR, Restriction, JC):
+        raise _lib.McpError(_lib.MCP_ERR_UNSUPPORTED,
+                            "parameter gradients cover the GTR, Restriction and JC models")
+    plan = _plan_for(d, x)
+    blv = get_branchlength_vector(d.tree)
+    ll, g, dP, dpi = plan.logl_grad_model(blv)
+    U, D, Uinv, mu = d.substitution_model(d.base_freq, d.substitution_rates)
+    out = {"branch_lengths": g, "rates": rate_gradient(U, D, Uinv, mu, blv, d.rates, dP),
+           "substitution_rates": np.zeros_like(d.substitution_rates)}
+    if d.substitution_model is GTR:
+        out["substitution_rates"], out["base_freq"] = gtr_gradient(
+            d.base_freq, d.substitution_rates, U, D, Uinv, mu, blv, d.rates, dP, dpi)
+    elif d.substitution_model is Restriction:
+        out["base_freq"] = restriction_gradient(d.base_freq, U, D, Uinv, mu, blv, d.rates, dP,
+                                                dpi)
+    else:
+        out["base_freq"] = np.asarray(dpi, dtype=np.float64).copy()  # JC: Q ignores pi
+    return ll, out
 
 
 # ---------------------------------------------------------------------------

@@ -128,3 +128,28 @@ def test_model_gradient_rejects_twenty_states():
     p = _plan(w)
     with pytest.raises(mcp.McpError):
         p.logl_grad_model(mcp.get_branchlength_vector(w.tree))
+
+
+@pytest.mark.parametrize("model", ["GTR", "JC"])
+def test_phylodist_parameter_gradients_vs_oracle_fd(model):
+    """gradlogpdf_params: branch lengths, category rates, base frequencies and
+    exchangeabilities of a PhyloDist, each against oracle finite differences."""
+    w = mcp.synthetic.make_workload("custom", seed=751, n=18, S=4, C=400, R=4)
+    fn = getattr(mcp, model)
+    d = mcp.PhyloDist(w.tree, w.pi, w.subst_rates, w.rates, fn)
+    print(f"phylodist parameter gradients {model}", flush=True)
+    ll, G = mcp.gradlogpdf_params(d, w.x)
+    blv = mcp.get_branchlength_vector(w.tree)
+
+    def ll_of(pi=w.pi, subst=w.subst_rates, rates=w.rates, b=blv):
+        U, D, Uinv, mu = fn(pi, subst)
+        return oracle.logl_grad(w.ta, w.x, U, D, Uinv, mu, rates, pi, b, grad=False)
+
+    assert rel_err(ll, ll_of()) < 1e-12
+    for name, x0, f in [("base_freq", w.pi, lambda q: ll_of(pi=q)),
+                        ("substitution_rates", w.subst_rates, lambda q: ll_of(subst=q)),
+                        ("rates", w.rates, lambda q: ll_of(rates=q)),
+                        ("branch_lengths", blv[:6], lambda q: ll_of(b=np.r_[q, blv[6:]]))]:
+        fd = _fd(f, x0)
+        got = G[name][:len(fd)]
+        assert np.max(np.abs(got - fd)) <= 1e-6 * max(1.0, np.max(np.abs(fd))), (name, got, fd)
