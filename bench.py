@@ -181,10 +181,10 @@ def bench_nuts(n_adapt, n_iter, cpu: bool, batch_chains: int = 64):
                        "sites, R=1, exponential(0.1) branch-length prior, NUTS target 0.8, "
                        "tree depth 5 (the reference's PNUTS test setting); ESS = min over tree "
                        "length and log posterior per chain, summed over chains"}
-    # a single chain's ESS from the reference estimator (batches of 100
-    # draws) needs many batches: the single GPU chain runs 10x the iterations,
-    # the 64-chain batch and the CPU baseline 2x
-    for key, nch, iters in (("gpu", 1, 10 * n_iter), ("gpu_batched", batch_chains, 2 * n_iter)):
+    # a chain's ESS from the reference estimator (batches of 100 draws) needs
+    # many batches: the GPU chains run 10x the iterations (20 batches each),
+    # the CPU baseline 2x
+    for key, nch, iters in (("gpu", 1, 10 * n_iter), ("gpu_batched", batch_chains, 10 * n_iter)):
         b = mcp.Batch([{"x_leaves": w.x[:, :, leaf - 1], "leaf_nums": leaf}],
                       [{"dataset": 0, "parent": w.ta.parent, "n_rates": 1}] * nch)
         for i in range(nch):
