@@ -225,6 +225,7 @@ struct mcp_engine {
     std::vector<int32_t> taskoff, blkjob;
     struct FusedLaunch {
         int S, cpt, sched_begin, nitems, nblocks;
+        bool tail;  // deep narrow pre-order levels, run after the per-level launches
         double bytes_f, bytes_a;  // algorithmic bytes of its post-/pre-order levels
     };
     // columns per thread chosen for each state count (-1: the built default);
@@ -728,10 +729,13 @@ static int rebuild_schedule(mcp_engine* e) {
     // and pre-order depths <= d0 (consecutive in time: the end of the
     // post-order pass and the start of the pre-order one).  LEVELS: none;
     // FUSED: all; HYBRID: the top levels of at most hybrid_width internal
-    // nodes, so narrow levels share one launch.
-    std::vector<int> h0(nitems), d0(nitems);
+    // nodes, so narrow levels share one launch, and a second fused launch
+    // ("tail") walks the pre-order depths >= d1 when every level from d1 down
+    // is that narrow too (at least two of them).
+    std::vector<int> h0(nitems), d0(nitems), d1(nitems);
     for (int i = 0; i < nitems; ++i) {
         const ItemHost& it = e->items[i];
+        d1[i] = it.max_depth + 1;
         if (e->schedule == MCP_SCHEDULE_LEVELS) {
             h0[i] = it.max_height + 1;
             d0[i] = -1;
@@ -751,6 +755,9 @@ static int rebuild_schedule(mcp_engine* e) {
             int dd = 0;
             while (dd <= it.max_depth && wd[dd] <= e->hybrid_width) ++dd;
             d0[i] = dd - 1;
+            int de = it.max_depth;
+            while (de > d0[i] && wd[de] <= e->hybrid_width) --de;
+            if (it.max_depth - de >= 2) d1[i] = de + 1;
         }
     }
     // node lists per (level) of the per-level launches
@@ -760,7 +767,7 @@ static int rebuild_schedule(mcp_engine* e) {
         for (int v = 0; v < it.N; ++v) {
             if (it.nodes[v].nchild == 0) continue;
             if (it.height[v] < h0[i]) by_h[it.height[v]].push_back({i, v});
-            if (it.depth[v] > d0[i]) by_d[it.depth[v]].push_back({i, v});
+            if (it.depth[v] > d0[i] && it.depth[v] < d1[i]) by_d[it.depth[v]].push_back({i, v});
         }
     }
     // algorithmic bytes of one job (the accounting of mcp_get_bytes, by job):
@@ -887,26 +894,60 @@ static int rebuild_schedule(mcp_engine* e) {
             const size_t k = std::find(svals.begin(), svals.end(), it.S) - svals.begin();
             per_s[k].push_back(I);
         }
+        // tails: the deep narrow pre-order levels of each item that has them
+        std::vector<std::vector<ItemSched>> tail_s(svals.size());
+        std::vector<double> tbytes(svals.size(), 0.0);
+        for (int i = 0; i < nitems; ++i) {
+            const ItemHost& it = e->items[i];
+            if (d1[i] > it.max_depth) continue;
+            std::vector<std::vector<int>> dv(it.max_depth + 1);
+            for (int v = 0; v < it.N; ++v)
+                if (it.nodes[v].nchild > 0) dv[it.depth[v]].push_back(v);
+            const size_t k = std::find(svals.begin(), svals.end(), it.S) - svals.begin();
+            ItemSched I{};
+            I.R = it.R;
+            I.ntiles = it.ntiles;
+            I.lf0 = I.lf1 = 0;
+            I.la0 = (int)la.size();
+            for (int dd = d1[i]; dd <= it.max_depth; ++dd) {
+                if (dv[dd].empty()) continue;
+                la.push_back((int32_t)ja.size());
+                for (int v : dv[dd]) {
+                    ja.push_back(make_job(i, v));
+                    tbytes[k] += job_bytes(true, i, v);
+                }
+            }
+            I.la1 = (int)la.size();
+            tail_s[k].push_back(I);
+        }
         lf.push_back((int32_t)jf.size());
         la.push_back((int32_t)ja.size());
-        for (size_t k = 0; k < svals.size(); ++k) {
+        auto add_fused = [&](const std::vector<ItemSched>& items_k, int S, double bf, double ba,
+                             bool tail) -> int {
             mcp_engine::FusedLaunch F{};
-            F.S = svals[k];
-            F.cpt = e->cpt(svals[k]);
-            F.bytes_f = fbytes[k].first;
-            F.bytes_a = fbytes[k].second;
+            F.S = S;
+            F.cpt = e->cpt(S);
+            F.bytes_f = bf;
+            F.bytes_a = ba;
+            F.tail = tail;
             F.sched_begin = (int)sched.size();
             long long blocks = 0;
-            for (ItemSched I : per_s[k]) {
+            for (ItemSched I : items_k) {
                 I.block_off = (int32_t)blocks;
                 blocks += (long long)I.R * I.ntiles;
                 sched.push_back(I);
             }
             if (blocks > 0x7fffffffLL) return fail(MCP_ERR_UNSUPPORTED, "too many tiles");
-            F.nitems = (int)per_s[k].size();
+            F.nitems = (int)items_k.size();
             F.nblocks = (int)blocks;
             if (F.nitems) e->fused.push_back(F);
-        }
+            return MCP_OK;
+        };
+        for (size_t k = 0; k < svals.size(); ++k)
+            if ((st = add_fused(per_s[k], svals[k], fbytes[k].first, fbytes[k].second, false)))
+                return st;
+        for (size_t k = 0; k < svals.size(); ++k)
+            if ((st = add_fused(tail_s[k], svals[k], 0.0, tbytes[k], true))) return st;
         if ((st = upload(e, e->dsched, sched.data(), sched.size() * sizeof(ItemSched)))) return st;
         if ((st = upload(e, e->dlvl_f, lf.data(), lf.size() * sizeof(int32_t)))) return st;
         if ((st = upload(e, e->dlvl_a, la.data(), la.size() * sizeof(int32_t)))) return st;
@@ -1238,6 +1279,7 @@ static int enqueue_eval(mcp_engine* e, bool grad, bool by_rate = false) {
     // the fused levels sit between the two passes (top of the post-order
     // pass, start of the pre-order pass)
     for (const auto& F : e->fused) {
+        if (F.tail) continue;
         launch_bytes = F.bytes_f + (grad ? F.bytes_a : 0.0);
         TIMED(grad ? 3 : 2,
               launch_fused(grad, F.S, F.cpt, e->dsched.as<ItemSched>() + F.sched_begin, F.nitems,
@@ -1245,7 +1287,7 @@ static int enqueue_eval(mcp_engine* e, bool grad, bool by_rate = false) {
                            e->djobs_a.as<LevelJob>(), e->dlvl_a.as<int32_t>(),
                            e->dparts.as<double>(), st));
     }
-    if (grad)
+    if (grad) {
         for (const Launch& L : e->adj) {
             launch_bytes = L.bytes;
             TIMED(3, launch_level(true, L.S, L.cpt, e->djobs_a.as<LevelJob>() + L.job_begin,
@@ -1253,6 +1295,15 @@ static int enqueue_eval(mcp_engine* e, bool grad, bool by_rate = false) {
                                   e->dblkjob.as<int32_t>() + L.blk_begin, L.ntasks, L.chunk,
                                   L.grid, e->dparts.as<double>(), st));
         }
+        for (const auto& F : e->fused) {
+            if (!F.tail) continue;
+            launch_bytes = F.bytes_a;
+            TIMED(3, launch_fused(true, F.S, F.cpt, e->dsched.as<ItemSched>() + F.sched_begin,
+                                  F.nitems, F.nblocks, e->waves, e->djobs_f.as<LevelJob>(),
+                                  e->dlvl_f.as<int32_t>(), e->djobs_a.as<LevelJob>(),
+                                  e->dlvl_a.as<int32_t>(), e->dparts.as<double>(), st));
+        }
+    }
     const int nops = (grad && !by_rate) ? (int)e->red.size() : e->n_ll_ops;
     const int nops2 = (grad && !by_rate) ? (int)e->red2.size() : e->n_ll_ops2;
     TIMED(4, launch_reduce(e->dred.as<RedOp>(), nops, e->dparts.as<double>(),
@@ -1385,10 +1436,11 @@ static int engine_eval_model(mcp_engine* e, const double* blv, double* ll, doubl
                              e->dblkjob.as<int32_t>() + L.blk_begin, L.ntasks, L.chunk, L.grid,
                              e->dparts.as<double>(), s));
     for (const auto& F : e->fused)
-        HIP_TRY(launch_fused(false, F.S, F.cpt, e->dsched.as<ItemSched>() + F.sched_begin,
-                             F.nitems, F.nblocks, e->waves, e->djobs_f.as<LevelJob>(),
-                             e->dlvl_f.as<int32_t>(), e->djobs_a.as<LevelJob>(),
-                             e->dlvl_a.as<int32_t>(), e->dparts.as<double>(), s));
+        if (!F.tail)
+            HIP_TRY(launch_fused(false, F.S, F.cpt, e->dsched.as<ItemSched>() + F.sched_begin,
+                                 F.nitems, F.nblocks, e->waves, e->djobs_f.as<LevelJob>(),
+                                 e->dlvl_f.as<int32_t>(), e->djobs_a.as<LevelJob>(),
+                                 e->dlvl_a.as<int32_t>(), e->dparts.as<double>(), s));
     for (const Launch& L : e->adj_mg)
         HIP_TRY(launch_level_mg(L.S, L.cpt, e->djobs_a.as<LevelJob>() + L.job_begin,
                                 e->dtaskoff.as<int32_t>() + L.taskoff_begin,
@@ -1484,15 +1536,22 @@ static int enqueue_leapfrog(mcp_engine* e) {
                              e->dblkjob.as<int32_t>() + L.blk_begin, L.ntasks, L.chunk, L.grid,
                              e->dparts.as<double>(), st));
     for (const auto& F : e->fused)
-        HIP_TRY(launch_fused(true, F.S, F.cpt, e->dsched.as<ItemSched>() + F.sched_begin, F.nitems,
-                             F.nblocks, e->waves, e->djobs_f.as<LevelJob>(),
-                             e->dlvl_f.as<int32_t>(), e->djobs_a.as<LevelJob>(),
-                             e->dlvl_a.as<int32_t>(), e->dparts.as<double>(), st));
+        if (!F.tail)
+            HIP_TRY(launch_fused(true, F.S, F.cpt, e->dsched.as<ItemSched>() + F.sched_begin,
+                                 F.nitems, F.nblocks, e->waves, e->djobs_f.as<LevelJob>(),
+                                 e->dlvl_f.as<int32_t>(), e->djobs_a.as<LevelJob>(),
+                                 e->dlvl_a.as<int32_t>(), e->dparts.as<double>(), st));
     for (const Launch& L : e->adj)
         HIP_TRY(launch_level(true, L.S, L.cpt, e->djobs_a.as<LevelJob>() + L.job_begin,
                              e->dtaskoff.as<int32_t>() + L.taskoff_begin,
                              e->dblkjob.as<int32_t>() + L.blk_begin, L.ntasks, L.chunk, L.grid,
                              e->dparts.as<double>(), st));
+    for (const auto& F : e->fused)
+        if (F.tail)
+            HIP_TRY(launch_fused(true, F.S, F.cpt, e->dsched.as<ItemSched>() + F.sched_begin,
+                                 F.nitems, F.nblocks, e->waves, e->djobs_f.as<LevelJob>(),
+                                 e->dlvl_f.as<int32_t>(), e->djobs_a.as<LevelJob>(),
+                                 e->dlvl_a.as<int32_t>(), e->dparts.as<double>(), st));
     HIP_TRY(launch_reduce(e->dred.as<RedOp>(), (int)e->red.size(), e->dparts.as<double>(),
                           e->dout.as<double>(), e->dredtmp.as<double>(), st));
     HIP_TRY(launch_reduce(e->dred2.as<RedOp>(), (int)e->red2.size(), e->dredtmp.as<double>(),

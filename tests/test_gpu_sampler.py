@@ -193,3 +193,32 @@ def test_native_delayed_acceptance_on_simudata():
     for k in range(d):
         se = np.hypot(S.mcse_bm(A[:, k]), S.mcse_bm(B[:, k]))
         assert abs(A[:, k].mean() - B[:, k].mean()) < 4.5 * se, k
+
+
+def test_tree_length_posterior_of_the_reference_pnuts_test():
+    """test/samplers/tree_samplers.jl:46-70: PNUTS on simudata under JC with a
+    CompoundDirichlet(1, 1, 0.1, 1) tree prior, target 0.8, tree depth 5; the
+    reference checks mean - sd <= 1.8 <= mean + sd for its tree-length summary.
+    Here: the native driver on the engine with the prior in the device
+    leapfrog (fixed topology and pi = 1/4: no NNI or pi moves)."""
+    nch = 2
+    t, x = load_fixture("simudata")
+    ta = mcp.tree_arrays(t)
+    U, D, Ui, mu = mcp.JC(np.ones(4) / 4)
+    leaf = ta.leaf_nums
+    b = mcp.Batch([{"x_leaves": x[:, :, leaf - 1], "leaf_nums": leaf}],
+                  [{"dataset": 0, "parent": ta.parent, "n_rates": 1}] * nch)
+    for i in range(nch):
+        b.set_model(i, mcp.Model(U, D, Ui, mu, [1.0], np.ones(4) / 4))
+        b.set_prior_compound_dirichlet(i, 1.0, 1.0, 0.1, 1.0)
+    b.use_graph(True)
+    blv = mcp.get_branchlength_vector(t)
+    nuts = S.NativeNUTS([blv] * nch, [123, 124], target=0.8, tree_depth=5)
+    nuts.use_engine(b)
+    nuts.run(500, adapt=True, keep_draws=False)
+    draws, lp, _, _, _ = nuts.run(1000)
+    tl = np.concatenate([d.sum(axis=1) for d in draws])
+    m, sd = tl.mean(), tl.std()
+    print(f"tree length posterior {m:.3f} +- {sd:.3f}", flush=True)
+    assert m - sd <= 1.8 <= m + sd, (m, sd)
+    assert np.all(np.isfinite(lp))
