@@ -5,7 +5,7 @@
 // one HIP stream, all device memory, pinned staging buffers and optional
 // HIP graphs of the evaluation.  Everything topology-derived is rebuilt on
 // the host (plan builder below) and uploaded; evaluation then is
-//   H2D branch lengths -> k_build_P -> k_tip_tables -> post-order levels ->
+//   H2D branch lengths -> k_build_P_tips -> post-order levels ->
 //   [pre-order levels] -> k_reduce -> D2H (ll, grad)
 // on the engine's stream.
 #include <hip/hip_runtime.h>
@@ -25,8 +25,8 @@
 #include "mcphylo_gpu.h"
 
 namespace mcp {
-hipError_t launch_build_P(const ModelDev*, const int32_t*, int, long long, hipStream_t);
-hipError_t launch_tip_tables(const ModelDev*, const int32_t*, int, long long, hipStream_t);
+hipError_t launch_build_P_tips(const ModelDev*, const int32_t*, const int32_t*, int, long long,
+                              long long, hipStream_t);
 hipError_t launch_level(bool, int, int, const LevelJob*, const int32_t*, const int32_t*, int, int,
                         int, double*, hipStream_t);
 hipError_t launch_fused(bool, int, int, const ItemSched*, int, int, int, const LevelJob*,
@@ -755,9 +755,9 @@ static int rebuild_schedule(mcp_engine* e) {
             int dd = 0;
             while (dd <= it.max_depth && wd[dd] <= e->hybrid_width) ++dd;
             d0[i] = dd - 1;
-            int de = it.max_depth;
-            while (de > d0[i] && wd[de] <= e->hybrid_width) --de;
-            if (it.max_depth - de >= 2) d1[i] = de + 1;
+            int de = it.max_depth, tail_levels = 0;
+            while (de > d0[i] && wd[de] <= e->hybrid_width) tail_levels += wd[de--] > 0;
+            if (tail_levels >= 2) d1[i] = de + 1;
         }
     }
     // node lists per (level) of the per-level launches
@@ -1265,10 +1265,9 @@ static int enqueue_eval(mcp_engine* e, bool grad, bool by_rate = false) {
     if (tm) HIP_TRY(hipEventRecord(e->ev_begin, st));
     HIP_TRY(hipMemcpyAsync(e->dblv.p, e->hblv.p, e->total_blv * sizeof(double),
                            hipMemcpyHostToDevice, st));
-    TIMED(1, launch_build_P(e->dmodels.as<ModelDev>(), e->delemP.as<int32_t>(),
-                            (int)e->items.size(), e->nelemP, st));
-    TIMED(1, launch_tip_tables(e->dmodels.as<ModelDev>(), e->delemT.as<int32_t>(),
-                               (int)e->items.size(), e->nelemT, st));
+    TIMED(1, launch_build_P_tips(e->dmodels.as<ModelDev>(), e->delemP.as<int32_t>(),
+                                 e->delemT.as<int32_t>(), (int)e->items.size(), e->nelemP,
+                                 e->nelemT, st));
     for (const Launch& L : e->fwd) {
         launch_bytes = L.bytes;
         TIMED(2, launch_level(false, L.S, L.cpt, e->djobs_f.as<LevelJob>() + L.job_begin,
@@ -1426,10 +1425,9 @@ static int engine_eval_model(mcp_engine* e, const double* blv, double* ll, doubl
     hipStream_t s = e->stream;
     HIP_TRY(hipMemcpyAsync(e->dblv.p, e->hblv.p, e->total_blv * sizeof(double),
                            hipMemcpyHostToDevice, s));
-    HIP_TRY(launch_build_P(e->dmodels.as<ModelDev>(), e->delemP.as<int32_t>(),
-                           (int)e->items.size(), e->nelemP, s));
-    HIP_TRY(launch_tip_tables(e->dmodels.as<ModelDev>(), e->delemT.as<int32_t>(),
-                              (int)e->items.size(), e->nelemT, s));
+    HIP_TRY(launch_build_P_tips(e->dmodels.as<ModelDev>(), e->delemP.as<int32_t>(),
+                                e->delemT.as<int32_t>(), (int)e->items.size(), e->nelemP,
+                                e->nelemT, s));
     for (const Launch& L : e->fwd)
         HIP_TRY(launch_level(false, L.S, L.cpt, e->djobs_f.as<LevelJob>() + L.job_begin,
                              e->dtaskoff.as<int32_t>() + L.taskoff_begin,
@@ -1527,9 +1525,8 @@ static int enqueue_leapfrog(mcp_engine* e) {
                            st));
     HIP_TRY(launch_lf_pre(e->dhmc_items.as<HmcItem>(), n, dx, dr, dg, ded, e->dblv.as<double>(),
                           64 * (int)std::max<size_t>(1, T), st));
-    HIP_TRY(launch_build_P(e->dmodels.as<ModelDev>(), e->delemP.as<int32_t>(), n, e->nelemP, st));
-    HIP_TRY(launch_tip_tables(e->dmodels.as<ModelDev>(), e->delemT.as<int32_t>(), n, e->nelemT,
-                              st));
+    HIP_TRY(launch_build_P_tips(e->dmodels.as<ModelDev>(), e->delemP.as<int32_t>(),
+                                e->delemT.as<int32_t>(), n, e->nelemP, e->nelemT, st));
     for (const Launch& L : e->fwd)
         HIP_TRY(launch_level(false, L.S, L.cpt, e->djobs_f.as<LevelJob>() + L.job_begin,
                              e->dtaskoff.as<int32_t>() + L.taskoff_begin,

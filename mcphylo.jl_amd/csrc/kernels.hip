@@ -2,7 +2,7 @@
 //
 // Kernel map (reference loop nest -> kernel):
 //   parallel_transition_prob / R_gemmturbo_large! / L_gemmturbo_large! /
-//   R_gemmturbo_large_ptg!  (VectorizedFunctions.jl:89-168)   -> k_build_P, k_tip_tables
+//   R_gemmturbo_large_ptg!  (VectorizedFunctions.jl:89-168)   -> k_build_P_tips
 //   comb_sum_product_loop! + by_max! + root_sum
 //   (VectorizedFunctions.jl:31-49,76-87,187-213;
 //    LikelihoodCalculator_Node.jl:82-114)                     -> k_fwd_level
@@ -61,59 +61,66 @@ __device__ __forceinline__ int find_item(const int32_t* __restrict__ off, int n,
     return lo;
 }
 
-// One thread per element (item, branch b, r, m, n).  elem_off[i] is the
-// first element of item i ((N-1)*R*S*S elements each).
-__global__ void k_build_P(const ModelDev* __restrict__ models, const int32_t* __restrict__ elem_off,
-                          int nitems) {
-    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= elem_off[nitems]) return;
-    const int i = find_item(elem_off, nitems, e);
-    const ModelDev& md = models[i];
-    const int S = md.S;
-    long long k = e - elem_off[i];
-    const int n = (int)(k % S); k /= S;
-    const int m = (int)(k % S); k /= S;
-    const int r = (int)(k % md.R);
-    const int b = (int)(k / md.R);
-    const double t = md.blv[b];
-    const double rho = md.rates[r];
-    double p = 0.0, dp = 0.0;
-    for (int q = 0; q < S; ++q) {
-        // argument order of VectorizedFunctions.jl:106,131: mu * blv * D * rate
-        const double ex = exp(md.mu * t * md.D[q] * rho);
-        const double u = md.U[m + S * q];
-        const double ui = md.Uinv[q + S * n];
-        p = fma(u * ex, ui, p);
-        dp = fma(u * md.D[q] * rho * md.mu * ex, ui, dp);
+// One launch builds every item's P, dP (one thread per element (item, branch
+// b, r, m, n); elemP[i] is the first element of item i, (N-1)*R*S*S each) and
+// the leaves' tip tables (one thread per element (item, branch b, r, code k,
+// state s), indices past the P elements; elemT[i] as elemP, only leaf
+// branches write): PT[leaf][r][k][s] = sum_s1 P_b[r][s][s1] tiptab[k][s1],
+// evaluated as sum_q U[s][q] e^{a_q} (Uinv tiptab_k)[q] so that no thread waits
+// for another's P.
+__global__ void k_build_P_tips(const ModelDev* __restrict__ models,
+                               const int32_t* __restrict__ elemP,
+                               const int32_t* __restrict__ elemT, int nitems) {
+    long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long nP = elemP[nitems];
+    if (e < nP) {
+        const int i = find_item(elemP, nitems, e);
+        const ModelDev& md = models[i];
+        const int S = md.S;
+        long long k = e - elemP[i];
+        const int n = (int)(k % S); k /= S;
+        const int m = (int)(k % S); k /= S;
+        const int r = (int)(k % md.R);
+        const int b = (int)(k / md.R);
+        const double t = md.blv[b];
+        const double rho = md.rates[r];
+        double p = 0.0, dp = 0.0;
+        for (int q = 0; q < S; ++q) {
+            // argument order of VectorizedFunctions.jl:106,131: mu * blv * D * rate
+            const double ex = exp(md.mu * t * md.D[q] * rho);
+            const double u = md.U[m + S * q];
+            const double ui = md.Uinv[q + S * n];
+            p = fma(u * ex, ui, p);
+            dp = fma(u * md.D[q] * rho * md.mu * ex, ui, dp);
+        }
+        const size_t o = (((size_t)b * md.R + r) * S + m) * S + n;
+        md.P[o] = p;
+        md.dP[o] = dp;
+        return;
     }
-    const size_t o = (((size_t)b * md.R + r) * S + m) * S + n;
-    md.P[o] = p;
-    md.dP[o] = dp;
-}
-
-// One thread per element (item, branch b, r, code k, state s); only leaf
-// branches write: PT[leaf][r][k][s] = sum_s1 P_b[r][s][s1] tiptab[k][s1].
-__global__ void k_tip_tables(const ModelDev* __restrict__ models, const int32_t* __restrict__ elem_off,
-                             int nitems) {
-    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= elem_off[nitems]) return;
-    const int i = find_item(elem_off, nitems, e);
+    e -= nP;
+    if (e >= elemT[nitems]) return;
+    const int i = find_item(elemT, nitems, e);
     const ModelDev& md = models[i];
     const int S = md.S;
-    long long k = e - elem_off[i];
+    long long k = e - elemT[i];
     const int s = (int)(k % S); k /= S;
     const int code = (int)(k % md.K); k /= md.K;
     const int r = (int)(k % md.R);
     const int b = (int)(k / md.R);
     const int leaf = md.leaf_of[b];
     if (leaf < 0) return;
-    const double* P = md.P + ((size_t)b * md.R + r) * S * S + (size_t)s * S;
-    const double* dP = md.dP + ((size_t)b * md.R + r) * S * S + (size_t)s * S;
+    const double t = md.blv[b];
+    const double rho = md.rates[r];
     const double* tv = md.tiptab + (size_t)code * S;
     double a = 0.0, da = 0.0;
-    for (int s1 = 0; s1 < S; ++s1) {
-        a = fma(P[s1], tv[s1], a);
-        da = fma(dP[s1], tv[s1], da);
+    for (int q = 0; q < S; ++q) {
+        double w = 0.0;
+        for (int s1 = 0; s1 < S; ++s1) w = fma(md.Uinv[q + S * s1], tv[s1], w);
+        const double ex = exp(md.mu * t * md.D[q] * rho);
+        const double u = md.U[s + S * q];
+        a = fma(u * ex, w, a);
+        da = fma(u * md.D[q] * rho * md.mu * ex, w, da);
     }
     const size_t o = (((size_t)leaf * md.R + r) * md.K + code) * S + s;
     md.PT[o] = a;
@@ -615,21 +622,14 @@ __global__ void __launch_bounds__(256) k_reduce(const RedOp* __restrict__ ops, i
 // launchers
 // ---------------------------------------------------------------------------
 
-hipError_t launch_build_P(const ModelDev* models, const int32_t* elem_off, int nitems,
-                          long long nelem, hipStream_t st) {
+hipError_t launch_build_P_tips(const ModelDev* models, const int32_t* elemP,
+                              const int32_t* elemT, int nitems, long long nelemP,
+                              long long nelemT, hipStream_t st) {
+    const long long nelem = nelemP + nelemT;
     if (nelem <= 0) return hipSuccess;
     const int tpb = 256;
-    hipLaunchKernelGGL(k_build_P, dim3((unsigned)((nelem + tpb - 1) / tpb)), dim3(tpb), 0, st,
-                       models, elem_off, nitems);
-    return hipGetLastError();
-}
-
-hipError_t launch_tip_tables(const ModelDev* models, const int32_t* elem_off, int nitems,
-                             long long nelem, hipStream_t st) {
-    if (nelem <= 0) return hipSuccess;
-    const int tpb = 256;
-    hipLaunchKernelGGL(k_tip_tables, dim3((unsigned)((nelem + tpb - 1) / tpb)), dim3(tpb), 0, st,
-                       models, elem_off, nitems);
+    hipLaunchKernelGGL(k_build_P_tips, dim3((unsigned)((nelem + tpb - 1) / tpb)), dim3(tpb), 0,
+                       st, models, elemP, elemT, nitems);
     return hipGetLastError();
 }
 

@@ -146,10 +146,14 @@ def test_phylodist_parameter_gradients_vs_oracle_fd(model):
         return oracle.logl_grad(w.ta, w.x, U, D, Uinv, mu, rates, pi, b, grad=False)
 
     assert rel_err(ll, ll_of()) < 1e-12
+    # the threaded oracle's sum changes in the last bits between calls, so a
+    # central difference with h = 1e-6 carries noise of about 1e-15 |ll| / 1e-6
+    noise = 1e-8 * abs(ll)
     for name, x0, f in [("base_freq", w.pi, lambda q: ll_of(pi=q)),
                         ("substitution_rates", w.subst_rates, lambda q: ll_of(subst=q)),
                         ("rates", w.rates, lambda q: ll_of(rates=q)),
                         ("branch_lengths", blv[:6], lambda q: ll_of(b=np.r_[q, blv[6:]]))]:
         fd = _fd(f, x0)
         got = G[name][:len(fd)]
-        assert np.max(np.abs(got - fd)) <= 1e-6 * max(1.0, np.max(np.abs(fd))), (name, got, fd)
+        tol = 1e-6 * max(1.0, np.max(np.abs(fd))) + noise
+        assert np.max(np.abs(got - fd)) <= tol, (name, got, fd)
