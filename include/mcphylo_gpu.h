@@ -134,7 +134,8 @@ int mcp_batch_set_topology(mcp_batch* batch, int32_t item, const int32_t* parent
 
 /* blv_concat: the items' branch-length vectors back to back (N_i - 1 each,
  * item order); ll: [n_items]; grad_concat: same layout as blv_concat.
- * logpdf(::MultiplePhyloDist) is the sum of ll; __logpdf is (ll, grad). */
+ * logpdf(::MultiplePhyloDist) (Phylodist.jl:281-288) is the sum of ll;
+ * __logpdf (Phylodist.jl:290-297) is the per-tree (ll, grad). */
 int mcp_batch_logl(mcp_batch* batch, const double* blv_concat, double* ll);
 int mcp_batch_logl_grad(mcp_batch* batch, const double* blv_concat, double* ll,
                         double* grad_concat);
@@ -199,7 +200,11 @@ int mcp_hmc_set_prior_params(mcp_batch* batch, int32_t item, int32_t kind, const
 int mcp_hmc_leapfrog(mcp_batch* batch, const double* eps_delta, double* x, double* r, double* g,
                      double* logpost, double* ll);
 
-/* ---- launch control ----------------------------------------------------- */
+/* ---- launch control ----------------------------------------------------- *
+ * No reference counterpart for this section: FelsensteinFunction
+ * (LikelihoodCalculator_Node.jl:3-114) walks the nodes one after another and
+ * threads each node's loops with @tturbo; here the launch schedule, tile
+ * widths, graphs and timing are the engine's own. */
 
 /* Capture the whole evaluation (upload of branch lengths, P/dP build, every
  * level, reductions, download) into HIP graphs that later calls replay; a
@@ -220,7 +225,9 @@ int mcp_use_graph(mcp_batch* batch, int32_t on);
  *                       levels above, and pre-order levels from the root
  *                       down to, the first level wider than the hybrid width);
  *                       the narrow levels are consecutive in time, between
- *                       the two passes. */
+ *                       the two passes.  A second fused launch walks the
+ *                       deepest pre-order levels when at least two of them,
+ *                       and every level below the first, are that narrow. */
 #define MCP_SCHEDULE_HYBRID 2
 int mcp_set_schedule(mcp_batch* batch, int32_t mode);
 
