@@ -8,6 +8,8 @@ test/likelihood/rates.jl:1-18, test/distributions/phylodist.jl:1-95.
 
 import os
 import re
+import shutil
+import subprocess
 
 import numpy as np
 import pytest
@@ -209,3 +211,30 @@ def test_no_cpu_fallback_when_library_missing(monkeypatch):
     monkeypatch.setattr(mcp._lib, "LIB_PATH", os.path.join(ROOT, "does-not-exist.so"))
     with pytest.raises(RuntimeError, match="not built"):
         mcp.load_library()
+
+
+def test_header_is_c99_and_a_c_program_links_against_the_library(tmp_path):
+    """The boundary is plain C: the header compiles as C99 with warnings as
+    errors, and a C caller links against the shared library and gets the
+    reference-style argument errors (no GPU call is made)."""
+    gcc = shutil.which("gcc")
+    if gcc is None:
+        pytest.skip("gcc not found")
+    inc = os.path.join(ROOT, "include")
+    src = tmp_path / "caller.c"
+    src.write_text(
+        '#include <stdio.h>\n#include <string.h>\n#include "mcphylo_gpu.h"\n'
+        "int main(void) {\n"
+        "    mcp_plan* p = 0;\n"
+        "    double ll = 0.0;\n"
+        "    if (mcp_plan_create(&p, NULL) != MCP_ERR_INVALID) return 1;\n"
+        '    if (strstr(mcp_last_error(), "null") == NULL) return 2;\n'
+        "    if (mcp_logl(NULL, NULL, &ll) != MCP_ERR_INVALID) return 3;\n"
+        '    printf("ok\\n");\n'
+        "    return 0;\n}\n")
+    exe = tmp_path / "caller"
+    lib = os.path.dirname(mcp._lib.LIB_PATH)
+    subprocess.run([gcc, "-std=c99", "-Wall", "-Wextra", "-Werror", "-I", inc, str(src), "-o",
+                    str(exe), "-L", lib, "-lmcphylo_hip", "-Wl,-rpath," + lib], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0 and out.stdout.strip() == "ok", (out.returncode, out.stderr)
