@@ -571,7 +571,9 @@ __global__ void __launch_bounds__(256) k_fused(const ItemSched* __restrict__ ite
     const int local = b - I.block_off;
     const int r = local / I.ntiles;
     const int tile = local - r * I.ntiles;
-    const int w = (int)(threadIdx.x >> 6);
+    // the wavefront index is uniform within a wavefront: readfirstlane keeps
+    // the job indices (and the job-record loads) scalar
+    const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int W = (int)(blockDim.x >> 6);
     for (int k = I.lf0; k < I.lf1; ++k) {
         const int j1 = lvl_f[k + 1];
