@@ -120,7 +120,7 @@ typedef struct mcp_batch_desc {
     int32_t n_items;
     const mcp_item_desc* items;
     int32_t device;
-    int32_t waves_per_block;        /* wavefronts per block of the fused schedule, 1..4;
+    int32_t waves_per_block;        /* wavefronts per block of the fused schedule, 1..8;
                                        0 = default (4) */
 } mcp_batch_desc;
 

@@ -209,7 +209,7 @@ struct Launch {
 
 struct mcp_engine {
     int device = 0;
-    int waves = 4;     // wavefronts per block of the fused kernel
+    int waves = 4;     // wavefronts per block of the fused kernel (1..8)
     int schedule = MCP_SCHEDULE_LEVELS;
     int hybrid_width = 4;  // widest level (internal nodes) run fused under HYBRID
     int num_cu = 256;
@@ -1060,8 +1060,11 @@ static int engine_create(mcp_engine** out, const mcp_batch_desc* d) {
     *out = nullptr;
     if (d->n_datasets < 1 || d->n_items < 1 || !d->datasets || !d->items)
         return fail(MCP_ERR_INVALID, "batch needs at least one dataset and one item");
-    const int waves = d->waves_per_block ? d->waves_per_block : 4;
-    if (waves < 1 || waves > 4) return fail(MCP_ERR_INVALID, "waves_per_block must be 1..4");
+#ifndef MCP_FUSED_WAVES
+#define MCP_FUSED_WAVES 4
+#endif
+    const int waves = d->waves_per_block ? d->waves_per_block : MCP_FUSED_WAVES;
+    if (waves < 1 || waves > 8) return fail(MCP_ERR_INVALID, "waves_per_block must be 1..8");
     int ndev = 0;
     HIP_TRY(hipGetDeviceCount(&ndev));
     if (d->device < 0 || d->device >= ndev)

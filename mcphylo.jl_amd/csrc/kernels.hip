@@ -555,7 +555,7 @@ __global__ void __launch_bounds__(64) k_level_mg(const LevelJob* __restrict__ jo
 // lanes covering the tile's columns).  A tile's partials are written and
 // read back by the same block, hence on the same XCD and its L2.
 template <int S, int CPT, bool GRAD>
-__global__ void __launch_bounds__(256) k_fused(const ItemSched* __restrict__ items, int nitems,
+__global__ void __launch_bounds__(512) k_fused(const ItemSched* __restrict__ items, int nitems,
                                                const LevelJob* __restrict__ jobs_f,
                                                const int32_t* __restrict__ lvl_f,
                                                const LevelJob* __restrict__ jobs_a,
