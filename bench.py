@@ -605,7 +605,9 @@ def main():
     pass_name = "pre-order" if dom == 3 else "post-order"
     kname = f"{pass_name} pass: k_level<{w.S},{cpt},{'true' if dom == 3 else 'false'}>"
     if sched == "hybrid":
-        kname += " per wide level + one k_fused launch over the narrow top levels"
+        kname = (f"{pass_name} pass: {nl} launch(es) -- k_level<{w.S},{cpt},true> per wide "
+                 f"level and k_fused<{w.S},{cpt},true> over the narrow levels (the first fused "
+                 "launch also walks the narrow top of the post-order pass)")
     elif sched == "fused":
         kname = f"k_fused<{w.S},{cpt},true> (both passes, one launch)"
     achieved = kbytes / (kms * 1e-3) / 1e9
