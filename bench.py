@@ -75,6 +75,8 @@ def bench_batched(data, steps, warmup, sample_trees, schedule):
     sched_rows = []
     if schedule == "auto":
         _, sched_rows = b.autotune_schedule(reps=2)
+        tiles, tune_rows = b.autotune(reps=2)
+        _, sched_rows = b.autotune_schedule(reps=2)
     sched, hwidth = b.schedule()
     log(f"  schedule: {sched}" + (f" (hybrid width {hwidth})" if sched == "hybrid" else ""))
     n_items, total_patterns = b.sizes()
@@ -569,6 +571,13 @@ def main():
     sched_rows = []
     if args.schedule == "auto":
         _, sched_rows = plan.autotune_schedule(reps=9)
+        if args.cols_per_thread <= 0:
+            # the best tile width can differ under the chosen schedule (the
+            # fused launches have one block per tile): tune it again there,
+            # then the schedule once more
+            tiles, tune_rows = plan.autotune(reps=5)
+            _, sched_rows = plan.autotune_schedule(reps=9)
+            log(f"tiles under the chosen schedule: {tiles}")
     sched, hwidth = plan.schedule()
     log(f"schedule: {sched}" + (f" (hybrid width {hwidth})" if sched == "hybrid" else ""))
     for _ in range(args.warmup):

@@ -235,6 +235,10 @@ int mcp_set_schedule(mcp_batch* batch, int32_t mode);
  * (default 4). */
 int mcp_set_hybrid_width(mcp_batch* batch, int32_t width);
 
+/* Wavefronts per block of the fused launches (1..8; the creation-time
+ * waves_per_block, default 4). */
+int mcp_set_fused_waves(mcp_batch* batch, int32_t waves);
+
 /* Tile width of the level kernels for state count S: a task is one
  * wavefront over 64 * cpt consecutive patterns (cpt = columns per lane).
  * Built candidates: cpt 1, 2, 4 for S = 2..4 (1 or 2 otherwise); for S = 20,

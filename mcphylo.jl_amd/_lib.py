@@ -44,7 +44,7 @@ EXPORTED = (
     "mcp_mlp_create", "mcp_mlp_destroy", "mcp_mlp_layout", "mcp_mlp_get_params",
     "mcp_mlp_set_params", "mcp_mlp_set_scaling", "mcp_mlp_set_data", "mcp_mlp_train",
     "mcp_mlp_param_grad", "mcp_mlp_predict", "mcp_mlp_leapfrog", "mcp_mlp_use_graph",
-    "mcp_mlp_train_graph",
+    "mcp_mlp_train_graph", "mcp_set_fused_waves",
     "mcp_logl_grad_by_rate", "mcp_batch_logl_grad_by_rate",
     "mcp_logl_grad_model", "mcp_batch_logl_grad_model",
     "mcp_nuts_create", "mcp_nuts_destroy", "mcp_nuts_use_batch", "mcp_nuts_use_mlp",
@@ -182,6 +182,7 @@ def load_library():
                                             _dp, _dp]),
         "mcp_mlp_use_graph": (ctypes.c_int, [vp, ctypes.c_int32]),
         "mcp_mlp_train_graph": (ctypes.c_int, [vp, ctypes.c_int32]),
+        "mcp_set_fused_waves": (ctypes.c_int, [vp, ctypes.c_int32]),
         "mcp_logl_grad_by_rate": (ctypes.c_int, [vp, _dp, _dp, _dp]),
         "mcp_batch_logl_grad_by_rate": (ctypes.c_int, [vp, _dp, _dp, _dp]),
         "mcp_logl_grad_model": (ctypes.c_int, [vp, _dp, _dp, _dp, _dp, _dp]),

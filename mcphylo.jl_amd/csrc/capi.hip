@@ -1806,6 +1806,16 @@ int mcp_set_hybrid_width(mcp_batch* b, int32_t width) {
     return b->schedule == MCP_SCHEDULE_HYBRID ? rebuild_schedule(b) : MCP_OK;
 }
 
+int mcp_set_fused_waves(mcp_batch* b, int32_t waves) {
+    if (!b) return fail(MCP_ERR_INVALID, "null handle");
+    if (waves < 1 || waves > 8) return fail(MCP_ERR_INVALID, "fused waves must be 1..8");
+    DeviceGuard guard(b->device);
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    b->waves = waves;
+    b->drop_graphs();  // captured fused launches hold the old block size
+    return MCP_OK;
+}
+
 int mcp_set_tiles(mcp_batch* b, int32_t S, int32_t cpt) {
     if (!b) return fail(MCP_ERR_INVALID, "null handle");
     if (!supported_states(S)) return fail(MCP_ERR_UNSUPPORTED, "no kernels for this state count");

@@ -70,6 +70,10 @@ class _Handle:
             check(_lib.load_library().mcp_set_hybrid_width(self._h, int(hybrid_width)))
         check(_lib.load_library().mcp_set_schedule(self._h, m))
 
+    def set_fused_waves(self, waves: int) -> None:
+        """Wavefronts per block of the fused launches (1..8)."""
+        check(_lib.load_library().mcp_set_fused_waves(self._h, int(waves)))
+
     def set_tiles(self, S: int, cpt: int) -> None:
         """Columns per thread of the level kernels for state count S (-1:
         built default; for S = 20, 0 = FP64-MFMA kernels)."""
