@@ -8,10 +8,34 @@ namespace mcp {
 // helpers
 // ---------------------------------------------------------------------------
 
+// One DPP move of a double (both 32-bit halves with the same pattern); lanes
+// of rows outside ROW_MASK receive 0.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ double dpp_move(double v) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)b, CTRL, ROW_MASK, 0xf, false);
+    const unsigned hi =
+        (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), CTRL, ROW_MASK, 0xf, false);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// Sum over the wavefront's 64 lanes, returned to every lane.  Every lane
+// must be active.  Register-only DPP steps (no LDS traffic, unlike lane
+// shuffles): quad butterflies, half-row and row mirrors give every lane its
+// 16-lane row sum, row broadcasts 15 and 31 carry the row sums up to lane
+// 63, whose total is read into a scalar register.  Fixed order, so the
+// result is the same on every run.
 __device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
+    v += dpp_move<0xB1, 0xf>(v);   // quad_perm [1,0,3,2]
+    v += dpp_move<0x4E, 0xf>(v);   // quad_perm [2,3,0,1]
+    v += dpp_move<0x141, 0xf>(v);  // row_half_mirror
+    v += dpp_move<0x140, 0xf>(v);  // row_mirror
+    v += dpp_move<0x142, 0xa>(v);  // row_bcast:15 into rows 1 and 3
+    v += dpp_move<0x143, 0xc>(v);  // row_bcast:31 into rows 2 and 3
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, 63);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), 63);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
 // Sum over the block; the result is valid in thread 0.  `lds` holds one
@@ -30,6 +54,12 @@ __device__ __forceinline__ double block_sum(double v, double* lds) {
     return s;
 }
 
+// A value the caller knows to be uniform across the wavefront, moved to a
+// scalar register.  Integer division is computed on the vector unit, so a
+// quotient of uniform operands lands in a VGPR; without this the addresses
+// derived from it turn wave-uniform loads into vector loads.
+__device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
 // Pointers read out of job records are generic; casting them to the global
 // address space lets the compiler emit global (not flat) loads and stores,
 // which wait on the vector-memory counter only.
@@ -42,6 +72,17 @@ __device__ __forceinline__ const MCP_GLOBAL T* gptr(const T* p) {
 template <class T>
 __device__ __forceinline__ MCP_GLOBAL T* gptr_w(T* p) {
     return (MCP_GLOBAL T*)p;
+}
+
+// Read-only operands of a launch (transition matrices, root frequencies,
+// written by an earlier launch): the constant address space tells the
+// compiler nothing in this launch stores to them, so wave-uniform reads
+// become scalar loads (SGPR operands) instead of vector loads of one address.
+#define MCP_CONST __attribute__((address_space(4)))
+
+template <class T>
+__device__ __forceinline__ const MCP_CONST T* cptr(const T* p) {
+    return (const MCP_CONST T*)p;
 }
 
 typedef double dvec2 __attribute__((ext_vector_type(2)));

@@ -169,7 +169,7 @@ __device__ __forceinline__ void child_fetch(const ChildRef& ch, const LevelJob& 
         const MCP_GLOBAL double* src = gptr(ch.F) + (size_t)r * J.Cp * S;
 #pragma unroll
         for (int k = 0; k < CPT; ++k) load_row<S>(src + (size_t)(c0 + 64 * k) * S, f[k]);
-        const MCP_GLOBAL double* __restrict__ P = gptr(ch.P) + (size_t)r * (S * S);
+        const MCP_CONST double* __restrict__ P = cptr(ch.P) + (size_t)r * (S * S);
 #pragma unroll
         for (int k = 0; k < CPT; ++k)
 #pragma unroll
@@ -205,7 +205,7 @@ __device__ __forceinline__ void fwd_task(const LevelJob& J, int r, int tile,
     double contrib = 0.0;
     if (J.is_root) {
         // root_sum: log sum_s pi_s Ftilde_s (VectorizedFunctions.jl:76-87)
-        const MCP_GLOBAL double* pi = gptr(J.pi);
+        const MCP_CONST double* pi = cptr(J.pi);
 #pragma unroll
         for (int k = 0; k < CPT; ++k) {
             const int c = c0 + 64 * k;
@@ -271,8 +271,8 @@ __device__ __forceinline__ double child_adjoint(const ChildRef& ch, const LevelJ
             contrib += c < J.C ? v : 0.0;
         }
     } else {
-        const MCP_GLOBAL double* __restrict__ P = gptr(ch.P) + (size_t)r * (S * S);
-        const MCP_GLOBAL double* __restrict__ dP = gptr(ch.dP) + (size_t)r * (S * S);
+        const MCP_CONST double* __restrict__ P = cptr(ch.P) + (size_t)r * (S * S);
+        const MCP_CONST double* __restrict__ dP = cptr(ch.dP) + (size_t)r * (S * S);
         MCP_GLOBAL double* Gdst = gptr_w(ch.G) + (size_t)r * J.Cp * S;
 #pragma unroll
         for (int k = 0; k < CPT; ++k) {
@@ -363,7 +363,7 @@ __device__ __forceinline__ void mg_root(const LevelJob& J, int r, int tile, int 
                                         const double (&Ft)[CPT][S],
                                         double* __restrict__ mparts) {
     const MCP_GLOBAL double* wts = gptr(J.weights);
-    const MCP_GLOBAL double* pi = gptr(J.pi);
+    const MCP_CONST double* pi = cptr(J.pi);
     double a[S];
 #pragma unroll
     for (int s = 0; s < S; ++s) a[s] = 0.0;
@@ -397,7 +397,7 @@ __device__ __forceinline__ void adj_task(const LevelJob& J, int r, int tile,
 #pragma unroll
         for (int k = 0; k < CPT; ++k) load_row<S>(src + (size_t)(c0 + 64 * k) * S, gp[k]);
     } else {
-        const MCP_GLOBAL double* pi = gptr(J.pi);
+        const MCP_CONST double* pi = cptr(J.pi);
 #pragma unroll
         for (int k = 0; k < CPT; ++k)
 #pragma unroll
@@ -513,8 +513,8 @@ __attribute__((amdgpu_waves_per_eu(ADJ ? MCP_ADJ_WAVES_PER_EU : MCP_FWD_WAVES_PE
         }
         const LevelJob& J = jobs[j];
         const int local = t - jbeg;
-        const int r = local / J.ntiles;
-        const int tile = local - r * J.ntiles;
+        const int r = uniform(local / J.ntiles);
+        const int tile = uniform(local - r * J.ntiles);
         if constexpr (ADJ) adj_task<S, CPT>(J, r, tile, parts);
         else fwd_task<S, CPT>(J, r, tile, parts);
     }
@@ -541,8 +541,8 @@ __global__ void __launch_bounds__(64) k_level_mg(const LevelJob* __restrict__ jo
         }
         const LevelJob& J = jobs[j];
         const int local = t - jbeg;
-        const int r = local / J.ntiles;
-        const int tile = local - r * J.ntiles;
+        const int r = uniform(local / J.ntiles);
+        const int tile = uniform(local - r * J.ntiles);
         adj_task<S, CPT, true>(J, r, tile, parts, mparts);
     }
 }
@@ -569,8 +569,8 @@ __global__ void __launch_bounds__(512) k_fused(const ItemSched* __restrict__ ite
     }
     const ItemSched I = items[lo];
     const int local = b - I.block_off;
-    const int r = local / I.ntiles;
-    const int tile = local - r * I.ntiles;
+    const int r = uniform(local / I.ntiles);
+    const int tile = uniform(local - r * I.ntiles);
     // the wavefront index is uniform within a wavefront: readfirstlane keeps
     // the job indices (and the job-record loads) scalar
     const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));

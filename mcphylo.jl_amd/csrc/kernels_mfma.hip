@@ -313,8 +313,8 @@ __attribute__((amdgpu_waves_per_eu(ADJ ? MCP_ADJ20_WAVES_PER_EU : 1))) k_level20
         }
         const LevelJob& J = jobs[j];
         const int local = t - jbeg;
-        const int r = local / J.ntiles;
-        const int tile = local - r * J.ntiles;
+        const int r = uniform(local / J.ntiles);
+        const int tile = uniform(local - r * J.ntiles);
         if constexpr (ADJ) adj_task20(J, r, tile, parts);
         else fwd_task20(J, r, tile, parts);
     }
