@@ -74,6 +74,12 @@ __device__ __forceinline__ MCP_GLOBAL T* gptr_w(T* p) {
     return (MCP_GLOBAL T*)p;
 }
 
+// Makes the listed wave-uniform values (fields of a job record) live in
+// scalar registers at this point.  The loads producing them cannot be sunk
+// to their first uses, so they issue together and are waited on once
+// instead of one dependent round trip per field.  Emits no instruction.
+#define MCP_PIN_SCALAR(...) asm volatile("" ::__VA_ARGS__)
+
 // Read-only operands of a launch (transition matrices, root frequencies,
 // written by an earlier launch): the constant address space tells the
 // compiler nothing in this launch stores to them, so wave-uniform reads

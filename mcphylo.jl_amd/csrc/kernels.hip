@@ -32,6 +32,7 @@
 // k_reduce in a fixed order, so results are bitwise reproducible run to run.
 #include <hip/hip_runtime.h>
 #include <math.h>
+#include <stddef.h>
 
 #include "device.h"
 #include "engine.h"
@@ -188,18 +189,34 @@ __device__ __forceinline__ void fwd_task(const LevelJob& J, int r, int tile,
                                          double* __restrict__ parts) {
     const int lane = (int)(threadIdx.x & 63);
     const int c0 = tile * 64 * CPT + lane;
+    // the record fields the task reads, fetched as one batch of scalar loads
+    const ChildRef a = J.kid[0];
+    const ChildRef b = J.kid[1];
+    MCP_PIN_SCALAR("s"(a.F), "s"(a.codes), "s"(a.P), "s"(b.F), "s"(b.codes), "s"(b.P),
+                   "s"(J.F), "s"(J.weights), "s"(J.nchild), "s"(J.is_root), "s"(J.lpart),
+                   "s"(J.Cp), "s"(J.K), "s"(J.C), "s"(J.ntiles));
     double acc[CPT][S];
-#pragma unroll
-    for (int k = 0; k < CPT; ++k)
-#pragma unroll
-        for (int s = 0; s < S; ++s) acc[k][s] = 1.0;
-    for (int q = 0; q < J.nchild; ++q) {
-        double f[CPT][S], m[CPT][S];
-        child_fetch<S, CPT>(child_of(J, q), J, r, c0, f, m);
+    if (J.nchild == 2) {
+        double fa[CPT][S], ma[CPT][S], fb[CPT][S], mb[CPT][S];
+        child_fetch<S, CPT>(a, J, r, c0, fa, ma);
+        child_fetch<S, CPT>(b, J, r, c0, fb, mb);
 #pragma unroll
         for (int k = 0; k < CPT; ++k)
 #pragma unroll
-            for (int s = 0; s < S; ++s) acc[k][s] *= m[k][s];
+            for (int s = 0; s < S; ++s) acc[k][s] = ma[k][s] * mb[k][s];
+    } else {
+#pragma unroll
+        for (int k = 0; k < CPT; ++k)
+#pragma unroll
+            for (int s = 0; s < S; ++s) acc[k][s] = 1.0;
+        for (int q = 0; q < J.nchild; ++q) {
+            double f[CPT][S], m[CPT][S];
+            child_fetch<S, CPT>(child_of(J, q), J, r, c0, f, m);
+#pragma unroll
+            for (int k = 0; k < CPT; ++k)
+#pragma unroll
+                for (int s = 0; s < S; ++s) acc[k][s] *= m[k][s];
+        }
     }
     const MCP_GLOBAL double* wts = gptr(J.weights);
     double contrib = 0.0;
@@ -391,6 +408,13 @@ __device__ __forceinline__ void adj_task(const LevelJob& J, int r, int tile,
                                          double* __restrict__ mparts = nullptr) {
     const int lane = (int)(threadIdx.x & 63);
     const int c0 = tile * 64 * CPT + lane;
+    // the record fields the task reads, fetched as one batch of scalar loads
+    const ChildRef a = J.kid[0];
+    const ChildRef b = J.kid[1];
+    MCP_PIN_SCALAR("s"(a.F), "s"(a.codes), "s"(a.P), "s"(a.dP), "s"(a.G), "s"(a.gpart),
+                   "s"(b.F), "s"(b.codes), "s"(b.P), "s"(b.dP), "s"(b.G), "s"(b.gpart),
+                   "s"(J.G), "s"(J.pi), "s"(J.weights), "s"(J.nchild), "s"(J.Cp), "s"(J.K),
+                   "s"(J.C), "s"(J.ntiles));
     double gp[CPT][S];
     if (J.G) {
         const MCP_GLOBAL double* src = gptr(J.G) + (size_t)r * J.Cp * S;
@@ -404,8 +428,6 @@ __device__ __forceinline__ void adj_task(const LevelJob& J, int r, int tile,
             for (int s = 0; s < S; ++s) gp[k][s] = pi[s];
     }
     if (J.nchild == 2) {
-        const ChildRef& a = J.kid[0];
-        const ChildRef& b = J.kid[1];
         double fa[CPT][S], ma[CPT][S], fb[CPT][S], mb[CPT][S], T[CPT][S];
         child_fetch<S, CPT>(a, J, r, c0, fa, ma);
         child_fetch<S, CPT>(b, J, r, c0, fb, mb);
@@ -547,6 +569,20 @@ __global__ void __launch_bounds__(64) k_level_mg(const LevelJob* __restrict__ jo
     }
 }
 
+// Reads one field on each 64-byte line of the record span the tasks read
+// (offsets 0x4 .. 0xb4: C, pi, kid[0].P, kid[1].P, kid[1].gpart are less
+// than 64 bytes apart), which leaves those lines in the scalar cache.
+static_assert(offsetof(LevelJob, C) <= 8 &&
+                  offsetof(LevelJob, pi) - offsetof(LevelJob, C) < 64 &&
+                  offsetof(LevelJob, kid) + offsetof(ChildRef, P) - offsetof(LevelJob, pi) < 64 &&
+                  sizeof(ChildRef) < 64 &&
+                  offsetof(ChildRef, gpart) - offsetof(ChildRef, P) < 64 &&
+                  offsetof(ChildRef, gpart) + 4 >= offsetof(ChildRef, mpart),
+              "warm_record touches one field per 64-byte line of the task span");
+__device__ __forceinline__ void warm_record(const LevelJob& J) {
+    MCP_PIN_SCALAR("s"(J.C), "s"(J.pi), "s"(J.kid[0].P), "s"(J.kid[1].P), "s"(J.kid[1].gpart));
+}
+
 // Fused schedule: every (item, category, pattern tile) is independent of all
 // other tiles through the whole evaluation, so one block walks all
 // post-order levels of its tile and then all pre-order levels, with a
@@ -575,15 +611,34 @@ __global__ void __launch_bounds__(512) k_fused(const ItemSched* __restrict__ ite
     // the job indices (and the job-record loads) scalar
     const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int W = (int)(blockDim.x >> 6);
+    // With the gradient, before each barrier a wavefront touches the record
+    // of its first task of the next level, so that the record's loads
+    // overlap the barrier (measured: faster with the pre-order pass, slower
+    // for the post-order pass alone).
     for (int k = I.lf0; k < I.lf1; ++k) {
         const int j1 = lvl_f[k + 1];
+        // the next level's job range, read with this level's bounds
+        int jn = 0, jn1 = 0;
+        if constexpr (GRAD) {
+            if (k + 1 < I.lf1) {
+                jn = j1 + w;
+                jn1 = lvl_f[k + 2];
+            } else if (I.la0 < I.la1) {
+                jn = lvl_a[I.la0] + w;
+                jn1 = lvl_a[I.la0 + 1];
+            }
+        }
         for (int j = lvl_f[k] + w; j < j1; j += W) fwd_task<S, CPT>(jobs_f[j], r, tile, parts);
+        if (jn < jn1) warm_record(k + 1 < I.lf1 ? jobs_f[jn] : jobs_a[jn]);
         __syncthreads();
     }
     if constexpr (GRAD) {
         for (int k = I.la0; k < I.la1; ++k) {
             const int j1 = lvl_a[k + 1];
+            const int jn = j1 + w;
+            const int jn1 = k + 1 < I.la1 ? lvl_a[k + 2] : 0;
             for (int j = lvl_a[k] + w; j < j1; j += W) adj_task<S, CPT>(jobs_a[j], r, tile, parts);
+            if (jn < jn1) warm_record(jobs_a[jn]);
             __syncthreads();
         }
     }
