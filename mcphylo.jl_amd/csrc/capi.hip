@@ -29,7 +29,7 @@ hipError_t launch_build_P_tips(const ModelDev*, const int32_t*, const int32_t*, 
                               long long, hipStream_t);
 hipError_t launch_level(bool, int, int, const LevelJob*, const int32_t*, const int32_t*, int, int,
                         int, double*, hipStream_t);
-hipError_t launch_fused(bool, int, int, const ItemSched*, int, int, int, const LevelJob*,
+hipError_t launch_fused(bool, int, int, const ItemSched*, int, int, int, size_t, const LevelJob*,
                         const int32_t*, const LevelJob*, const int32_t*, double*, hipStream_t);
 int default_cpt(int S);
 bool cpt_available(int S, int cpt);
@@ -156,6 +156,11 @@ struct NodeInfo {
     int32_t is_root;
 };
 
+// Largest tip-code stage (bytes of LDS per block) of a fused launch: at 64
+// patterns per tile, 512 leaves.  Larger items read their codes from global
+// memory.
+constexpr size_t kLdsCodesMax = 32 * 1024;
+
 struct DatasetHost {
     int S = 0, C = 0, Cp = 0, K = 0, nleaves = 0;
     std::vector<int32_t> leaf_nums;          // row j of codes is leaf leaf_nums[j]
@@ -227,6 +232,7 @@ struct mcp_engine {
     struct FusedLaunch {
         int S, cpt, sched_begin, nitems, nblocks;
         bool tail;  // deep narrow pre-order levels, run after the per-level launches
+        size_t lds_bytes;  // tip codes staged per block (max over the launch's items)
         double bytes_f, bytes_a;  // algorithmic bytes of its post-/pre-order levels
     };
     // columns per thread chosen for each state count (-1: the built default);
@@ -656,6 +662,7 @@ static int rebuild_schedule(mcp_engine* e) {
             c.P = e->dPT.as<double>() + it.PT_off + (size_t)n.leaf * it.R * d.K * it.S;
             c.dP = e->ddPT.as<double>() + it.PT_off + (size_t)n.leaf * it.R * d.K * it.S;
             c.G = nullptr;
+            c.leaf = n.leaf;
         } else {
             c.F = e->dF.as<double>() + it.F_off + (size_t)n.fslot * slab;
             c.codes = nullptr;
@@ -663,6 +670,7 @@ static int rebuild_schedule(mcp_engine* e) {
             c.dP = e->ddP.as<double>() + it.P_off + (size_t)u * it.R * it.S * it.S;
             double* gb = n.gbuf ? e->dG1.as<double>() + it.G_off[1] : e->dG0.as<double>() + it.G_off[0];
             c.G = gb + (size_t)n.gslot * slab;
+            c.leaf = -1;
         }
         c.gpart = it.parts_off + n.gpart;
         c.mpart = it.mg_off < 0 ? -1
@@ -861,10 +869,12 @@ static int rebuild_schedule(mcp_engine* e) {
         for (int i = 0; i < nitems; ++i) {
             const ItemHost& it = e->items[i];
             const DatasetHost& d = e->ds[it.dataset];
-            (void)d;
             ItemSched I{};
             I.R = it.R;
             I.ntiles = it.ntiles;
+            I.codes = e->dcodes.as<uint8_t>() + d.codes_off;
+            I.Cp = d.Cp;
+            I.lds_leaves = d.nleaves;  // settled per launch in add_fused
             std::vector<std::vector<int>> hv(it.max_height + 1), dv(it.max_depth + 1);
             for (int v = 0; v < it.N; ++v) {
                 if (it.nodes[v].nchild == 0) continue;
@@ -905,9 +915,13 @@ static int rebuild_schedule(mcp_engine* e) {
             for (int v = 0; v < it.N; ++v)
                 if (it.nodes[v].nchild > 0) dv[it.depth[v]].push_back(v);
             const size_t k = std::find(svals.begin(), svals.end(), it.S) - svals.begin();
+            const DatasetHost& d = e->ds[it.dataset];
             ItemSched I{};
             I.R = it.R;
             I.ntiles = it.ntiles;
+            I.codes = e->dcodes.as<uint8_t>() + d.codes_off;
+            I.Cp = d.Cp;
+            I.lds_leaves = d.nleaves;
             I.lf0 = I.lf1 = 0;
             I.la0 = (int)la.size();
             for (int dd = d1[i]; dd <= it.max_depth; ++dd) {
@@ -932,10 +946,19 @@ static int rebuild_schedule(mcp_engine* e) {
             F.bytes_a = ba;
             F.tail = tail;
             F.sched_begin = (int)sched.size();
+            F.lds_bytes = 0;
+            // an item stages its tip codes in LDS when its rows fit kLdsCodesMax
+            // and its tiles stay within the padded pattern range
+            const int tcols = 64 * std::max(1, F.cpt);
             long long blocks = 0;
             for (ItemSched I : items_k) {
                 I.block_off = (int32_t)blocks;
                 blocks += (long long)I.R * I.ntiles;
+                const size_t bytes = (size_t)I.lds_leaves * tcols;
+                if (I.lds_leaves > 0 && bytes <= kLdsCodesMax && (long long)I.ntiles * tcols <= I.Cp)
+                    F.lds_bytes = std::max(F.lds_bytes, bytes);
+                else
+                    I.lds_leaves = 0;
                 sched.push_back(I);
             }
             if (blocks > 0x7fffffffLL) return fail(MCP_ERR_UNSUPPORTED, "too many tiles");
@@ -1286,7 +1309,7 @@ static int enqueue_eval(mcp_engine* e, bool grad, bool by_rate = false) {
         launch_bytes = F.bytes_f + (grad ? F.bytes_a : 0.0);
         TIMED(grad ? 3 : 2,
               launch_fused(grad, F.S, F.cpt, e->dsched.as<ItemSched>() + F.sched_begin, F.nitems,
-                           F.nblocks, e->waves, e->djobs_f.as<LevelJob>(), e->dlvl_f.as<int32_t>(),
+                           F.nblocks, e->waves, F.lds_bytes, e->djobs_f.as<LevelJob>(), e->dlvl_f.as<int32_t>(),
                            e->djobs_a.as<LevelJob>(), e->dlvl_a.as<int32_t>(),
                            e->dparts.as<double>(), st));
     }
@@ -1302,7 +1325,7 @@ static int enqueue_eval(mcp_engine* e, bool grad, bool by_rate = false) {
             if (!F.tail) continue;
             launch_bytes = F.bytes_a;
             TIMED(3, launch_fused(true, F.S, F.cpt, e->dsched.as<ItemSched>() + F.sched_begin,
-                                  F.nitems, F.nblocks, e->waves, e->djobs_f.as<LevelJob>(),
+                                  F.nitems, F.nblocks, e->waves, F.lds_bytes, e->djobs_f.as<LevelJob>(),
                                   e->dlvl_f.as<int32_t>(), e->djobs_a.as<LevelJob>(),
                                   e->dlvl_a.as<int32_t>(), e->dparts.as<double>(), st));
         }
@@ -1440,7 +1463,7 @@ static int engine_eval_model(mcp_engine* e, const double* blv, double* ll, doubl
     for (const auto& F : e->fused)
         if (!F.tail)
             HIP_TRY(launch_fused(false, F.S, F.cpt, e->dsched.as<ItemSched>() + F.sched_begin,
-                                 F.nitems, F.nblocks, e->waves, e->djobs_f.as<LevelJob>(),
+                                 F.nitems, F.nblocks, e->waves, F.lds_bytes, e->djobs_f.as<LevelJob>(),
                                  e->dlvl_f.as<int32_t>(), e->djobs_a.as<LevelJob>(),
                                  e->dlvl_a.as<int32_t>(), e->dparts.as<double>(), s));
     for (const Launch& L : e->adj_mg)
@@ -1539,7 +1562,7 @@ static int enqueue_leapfrog(mcp_engine* e) {
     for (const auto& F : e->fused)
         if (!F.tail)
             HIP_TRY(launch_fused(true, F.S, F.cpt, e->dsched.as<ItemSched>() + F.sched_begin,
-                                 F.nitems, F.nblocks, e->waves, e->djobs_f.as<LevelJob>(),
+                                 F.nitems, F.nblocks, e->waves, F.lds_bytes, e->djobs_f.as<LevelJob>(),
                                  e->dlvl_f.as<int32_t>(), e->djobs_a.as<LevelJob>(),
                                  e->dlvl_a.as<int32_t>(), e->dparts.as<double>(), st));
     for (const Launch& L : e->adj)
@@ -1550,7 +1573,7 @@ static int enqueue_leapfrog(mcp_engine* e) {
     for (const auto& F : e->fused)
         if (F.tail)
             HIP_TRY(launch_fused(true, F.S, F.cpt, e->dsched.as<ItemSched>() + F.sched_begin,
-                                 F.nitems, F.nblocks, e->waves, e->djobs_f.as<LevelJob>(),
+                                 F.nitems, F.nblocks, e->waves, F.lds_bytes, e->djobs_f.as<LevelJob>(),
                                  e->dlvl_f.as<int32_t>(), e->djobs_a.as<LevelJob>(),
                                  e->dlvl_a.as<int32_t>(), e->dparts.as<double>(), st));
     HIP_TRY(launch_reduce(e->dred.as<RedOp>(), (int)e->red.size(), e->dparts.as<double>(),

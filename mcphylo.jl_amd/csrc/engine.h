@@ -28,6 +28,8 @@ struct ChildRef {
     int32_t gpart;         // offset of the branch's gradient partial sums
     int32_t mpart;         // offset of the branch's d logL / dP partial sums (model-gradient
                            // buffer, layout [r][S*S][tile]), -1 when not laid out
+    int32_t leaf;          // leaf child: its row of the dataset's code matrix; internal: -1
+    int32_t pad_;
 };
 
 constexpr int kInlineChildren = 3;
@@ -74,6 +76,9 @@ struct ItemSched {
     int32_t R, ntiles;
     int32_t lf0, lf1;      // post-order levels: [lf0, lf1) into lvl_f (lvl_f[k] = first job)
     int32_t la0, la1;      // pre-order levels:  [la0, la1) into lvl_a
+    int32_t lds_leaves;    // code rows each block stages in LDS (0: leaves read global codes)
+    const uint8_t* codes;  // the item's dataset code matrix [leaf][Cp]
+    int32_t Cp;
     int32_t pad_;
 };
 

@@ -146,6 +146,27 @@ __device__ __forceinline__ void select_row(const MCP_GLOBAL double* __restrict__
     for (int s = 0; s < S; ++s) out[s] = row[s];
 }
 
+// Tip codes the fused kernel stages in LDS: [leaf][64 * CPT] bytes, the
+// block's tile of every code row of its item's dataset.  A task's leaf child
+// then reads its code bytes from LDS (lrow = 64 * CPT) instead of global
+// memory (lrow = 0), which takes a cold HBM round trip off every level step
+// that has a leaf child.
+extern __shared__ uint8_t lds_codes[];
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+template <int CPT>
+__device__ __forceinline__ void leaf_codes(const ChildRef& ch, int c0, int lrow, int (&code)[CPT]) {
+    if (lrow) {
+        const int lane = (int)(threadIdx.x & 63);
+#pragma unroll
+        for (int k = 0; k < CPT; ++k) code[k] = (int)lds_codes[ch.leaf * lrow + lane + 64 * k];
+    } else {
+        const MCP_GLOBAL uint8_t* codes = gptr(ch.codes);
+#pragma unroll
+        for (int k = 0; k < CPT; ++k) code[k] = (int)codes[c0 + 64 * k];
+    }
+}
+
 // Columns of a task: lane l owns c0 + 64 k (k < CPT), c0 = tile base + l.
 // Slabs, codes and weights are allocated over the padded pattern range, so
 // loads and stores of padding columns stay in bounds and are issued
@@ -157,13 +178,11 @@ __device__ __forceinline__ void select_row(const MCP_GLOBAL double* __restrict__
 // `f`), or the leaf's precomputed row PT[r][code] (`f` untouched).
 template <int S, int CPT>
 __device__ __forceinline__ void child_fetch(const ChildRef& ch, const LevelJob& J, int r, int c0,
-                                            double (&f)[CPT][S], double (&m)[CPT][S]) {
+                                            int lrow, double (&f)[CPT][S], double (&m)[CPT][S]) {
     if (ch.codes) {
         const MCP_GLOBAL double* pt = gptr(ch.P) + (size_t)r * J.K * S;
-        const MCP_GLOBAL uint8_t* codes = gptr(ch.codes);
         int code[CPT];
-#pragma unroll
-        for (int k = 0; k < CPT; ++k) code[k] = (int)codes[c0 + 64 * k];
+        leaf_codes<CPT>(ch, c0, lrow, code);
 #pragma unroll
         for (int k = 0; k < CPT; ++k) select_row<S>(pt, code[k], m[k]);
     } else {
@@ -186,20 +205,20 @@ __device__ __forceinline__ void child_fetch(const ChildRef& ch, const LevelJob& 
 // post-order: partial of the job's node for the thread's columns
 template <int S, int CPT>
 __device__ __forceinline__ void fwd_task(const LevelJob& J, int r, int tile,
-                                         double* __restrict__ parts) {
+                                         double* __restrict__ parts, int lrow = 0) {
     const int lane = (int)(threadIdx.x & 63);
     const int c0 = tile * 64 * CPT + lane;
     // the record fields the task reads, fetched as one batch of scalar loads
     const ChildRef a = J.kid[0];
     const ChildRef b = J.kid[1];
-    MCP_PIN_SCALAR("s"(a.F), "s"(a.codes), "s"(a.P), "s"(b.F), "s"(b.codes), "s"(b.P),
-                   "s"(J.F), "s"(J.weights), "s"(J.nchild), "s"(J.is_root), "s"(J.lpart),
-                   "s"(J.Cp), "s"(J.K), "s"(J.C), "s"(J.ntiles));
+    MCP_PIN_SCALAR("s"(a.F), "s"(a.codes), "s"(a.P), "s"(a.leaf), "s"(b.F), "s"(b.codes),
+                   "s"(b.P), "s"(b.leaf), "s"(J.F), "s"(J.weights), "s"(J.nchild),
+                   "s"(J.is_root), "s"(J.lpart), "s"(J.Cp), "s"(J.K), "s"(J.C), "s"(J.ntiles));
     double acc[CPT][S];
     if (J.nchild == 2) {
         double fa[CPT][S], ma[CPT][S], fb[CPT][S], mb[CPT][S];
-        child_fetch<S, CPT>(a, J, r, c0, fa, ma);
-        child_fetch<S, CPT>(b, J, r, c0, fb, mb);
+        child_fetch<S, CPT>(a, J, r, c0, lrow, fa, ma);
+        child_fetch<S, CPT>(b, J, r, c0, lrow, fb, mb);
 #pragma unroll
         for (int k = 0; k < CPT; ++k)
 #pragma unroll
@@ -211,7 +230,7 @@ __device__ __forceinline__ void fwd_task(const LevelJob& J, int r, int tile,
             for (int s = 0; s < S; ++s) acc[k][s] = 1.0;
         for (int q = 0; q < J.nchild; ++q) {
             double f[CPT][S], m[CPT][S];
-            child_fetch<S, CPT>(child_of(J, q), J, r, c0, f, m);
+            child_fetch<S, CPT>(child_of(J, q), J, r, c0, lrow, f, m);
 #pragma unroll
             for (int k = 0; k < CPT; ++k)
 #pragma unroll
@@ -266,17 +285,18 @@ __device__ __forceinline__ double child_adjoint(const ChildRef& ch, const LevelJ
                                                 const double (&T)[CPT][S],
                                                 const double (&f)[CPT][S],
                                                 const double (&m)[CPT][S],
-                                                double (&colv)[CPT]) {
+                                                double (&colv)[CPT], int lrow) {
     const MCP_GLOBAL double* wts = gptr(J.weights);
     double contrib = 0.0;
     if (ch.codes) {
         const MCP_GLOBAL double* dpt = gptr(ch.dP) + (size_t)r * J.K * S;
-        const MCP_GLOBAL uint8_t* codes = gptr(ch.codes);
+        int code[CPT];
+        leaf_codes<CPT>(ch, c0, lrow, code);
 #pragma unroll
         for (int k = 0; k < CPT; ++k) {
             const int c = c0 + 64 * k;
             double row[S];
-            select_row<S>(dpt, (int)codes[c], row);
+            select_row<S>(dpt, code[k], row);
             double num = 0.0, col = 0.0;
 #pragma unroll
             for (int s = 0; s < S; ++s) {
@@ -405,16 +425,16 @@ __device__ __forceinline__ void mg_root(const LevelJob& J, int r, int tile, int 
 template <int S, int CPT, bool MG = false>
 __device__ __forceinline__ void adj_task(const LevelJob& J, int r, int tile,
                                          double* __restrict__ parts,
-                                         double* __restrict__ mparts = nullptr) {
+                                         double* __restrict__ mparts = nullptr, int lrow = 0) {
     const int lane = (int)(threadIdx.x & 63);
     const int c0 = tile * 64 * CPT + lane;
     // the record fields the task reads, fetched as one batch of scalar loads
     const ChildRef a = J.kid[0];
     const ChildRef b = J.kid[1];
     MCP_PIN_SCALAR("s"(a.F), "s"(a.codes), "s"(a.P), "s"(a.dP), "s"(a.G), "s"(a.gpart),
-                   "s"(b.F), "s"(b.codes), "s"(b.P), "s"(b.dP), "s"(b.G), "s"(b.gpart),
-                   "s"(J.G), "s"(J.pi), "s"(J.weights), "s"(J.nchild), "s"(J.Cp), "s"(J.K),
-                   "s"(J.C), "s"(J.ntiles));
+                   "s"(a.leaf), "s"(b.F), "s"(b.codes), "s"(b.P), "s"(b.dP), "s"(b.G),
+                   "s"(b.gpart), "s"(b.leaf), "s"(J.G), "s"(J.pi), "s"(J.weights), "s"(J.nchild),
+                   "s"(J.Cp), "s"(J.K), "s"(J.C), "s"(J.ntiles));
     double gp[CPT][S];
     if (J.G) {
         const MCP_GLOBAL double* src = gptr(J.G) + (size_t)r * J.Cp * S;
@@ -429,20 +449,20 @@ __device__ __forceinline__ void adj_task(const LevelJob& J, int r, int tile,
     }
     if (J.nchild == 2) {
         double fa[CPT][S], ma[CPT][S], fb[CPT][S], mb[CPT][S], T[CPT][S];
-        child_fetch<S, CPT>(a, J, r, c0, fa, ma);
-        child_fetch<S, CPT>(b, J, r, c0, fb, mb);
+        child_fetch<S, CPT>(a, J, r, c0, lrow, fa, ma);
+        child_fetch<S, CPT>(b, J, r, c0, lrow, fb, mb);
 #pragma unroll
         for (int k = 0; k < CPT; ++k)
 #pragma unroll
             for (int s = 0; s < S; ++s) T[k][s] = gp[k][s] * mb[k][s];
         double colv[CPT];
-        const double ca = child_adjoint<S, CPT>(a, J, r, c0, T, fa, ma, colv);
+        const double ca = child_adjoint<S, CPT>(a, J, r, c0, T, fa, ma, colv, lrow);
         if constexpr (MG) mg_branch<S, CPT>(a, J, r, tile, c0, T, fa, colv, mparts);
 #pragma unroll
         for (int k = 0; k < CPT; ++k)
 #pragma unroll
             for (int s = 0; s < S; ++s) T[k][s] = gp[k][s] * ma[k][s];
-        const double cb = child_adjoint<S, CPT>(b, J, r, c0, T, fb, mb, colv);
+        const double cb = child_adjoint<S, CPT>(b, J, r, c0, T, fb, mb, colv, lrow);
         if constexpr (MG) {
             mg_branch<S, CPT>(b, J, r, tile, c0, T, fb, colv, mparts);
             if (!J.G) {
@@ -472,15 +492,15 @@ __device__ __forceinline__ void adj_task(const LevelJob& J, int r, int tile,
         for (int q2 = 0; q2 < J.nchild; ++q2) {
             if (q2 == q) continue;
             double fw[CPT][S], mw[CPT][S];
-            child_fetch<S, CPT>(child_of(J, q2), J, r, c0, fw, mw);
+            child_fetch<S, CPT>(child_of(J, q2), J, r, c0, lrow, fw, mw);
 #pragma unroll
             for (int k = 0; k < CPT; ++k)
 #pragma unroll
                 for (int s = 0; s < S; ++s) T[k][s] *= mw[k][s];
         }
-        child_fetch<S, CPT>(u, J, r, c0, fu, mu_);
+        child_fetch<S, CPT>(u, J, r, c0, lrow, fu, mu_);
         double colv[CPT];
-        const double cu = child_adjoint<S, CPT>(u, J, r, c0, T, fu, mu_, colv);
+        const double cu = child_adjoint<S, CPT>(u, J, r, c0, T, fu, mu_, colv, lrow);
         if constexpr (MG) mg_branch<S, CPT>(u, J, r, tile, c0, T, fu, colv, mparts);
         const double su = wave_sum(cu);
         if (lane == 0) parts[u.gpart + r * J.ntiles + tile] = su;
@@ -494,7 +514,7 @@ __device__ __forceinline__ void adj_task(const LevelJob& J, int r, int tile,
                 for (int s = 0; s < S; ++s) Ft[k][s] = 1.0;
             for (int q = 0; q < J.nchild; ++q) {
                 double fw[CPT][S], mw[CPT][S];
-                child_fetch<S, CPT>(child_of(J, q), J, r, c0, fw, mw);
+                child_fetch<S, CPT>(child_of(J, q), J, r, c0, lrow, fw, mw);
 #pragma unroll
                 for (int k = 0; k < CPT; ++k)
 #pragma unroll
@@ -570,17 +590,18 @@ __global__ void __launch_bounds__(64) k_level_mg(const LevelJob* __restrict__ jo
 }
 
 // Reads one field on each 64-byte line of the record span the tasks read
-// (offsets 0x4 .. 0xb4: C, pi, kid[0].P, kid[1].P, kid[1].gpart are less
+// (from C to kid[1].leaf: C, pi, kid[0].P, kid[1].P, kid[1].leaf are less
 // than 64 bytes apart), which leaves those lines in the scalar cache.
 static_assert(offsetof(LevelJob, C) <= 8 &&
                   offsetof(LevelJob, pi) - offsetof(LevelJob, C) < 64 &&
                   offsetof(LevelJob, kid) + offsetof(ChildRef, P) - offsetof(LevelJob, pi) < 64 &&
                   sizeof(ChildRef) < 64 &&
-                  offsetof(ChildRef, gpart) - offsetof(ChildRef, P) < 64 &&
-                  offsetof(ChildRef, gpart) + 4 >= offsetof(ChildRef, mpart),
+                  offsetof(ChildRef, leaf) - offsetof(ChildRef, P) < 64 &&
+                  offsetof(ChildRef, leaf) > offsetof(ChildRef, gpart) &&
+                  offsetof(ChildRef, leaf) > offsetof(ChildRef, G),
               "warm_record touches one field per 64-byte line of the task span");
 __device__ __forceinline__ void warm_record(const LevelJob& J) {
-    MCP_PIN_SCALAR("s"(J.C), "s"(J.pi), "s"(J.kid[0].P), "s"(J.kid[1].P), "s"(J.kid[1].gpart));
+    MCP_PIN_SCALAR("s"(J.C), "s"(J.pi), "s"(J.kid[0].P), "s"(J.kid[1].P), "s"(J.kid[1].leaf));
 }
 
 // Fused schedule: every (item, category, pattern tile) is independent of all
@@ -611,6 +632,21 @@ __global__ void __launch_bounds__(512) k_fused(const ItemSched* __restrict__ ite
     // the job indices (and the job-record loads) scalar
     const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int W = (int)(blockDim.x >> 6);
+    // stage the tile's code bytes of every leaf row (16-byte chunks; rows are
+    // Cp bytes apart, Cp and the tile base are multiples of 64 * CPT)
+    const int lrow = I.lds_leaves > 0 ? 64 * CPT : 0;
+    if (lrow) {
+        constexpr int kChunks = 64 * CPT / 16;
+        const MCP_GLOBAL uint8_t* src = gptr(I.codes) + (size_t)tile * 64 * CPT;
+        const int n = I.lds_leaves * kChunks;
+        for (int i = (int)threadIdx.x; i < n; i += (int)blockDim.x) {
+            const int l = i / kChunks;
+            const int o = (i - l * kChunks) * 16;
+            *reinterpret_cast<u32x4*>(lds_codes + l * 64 * CPT + o) =
+                *reinterpret_cast<const MCP_GLOBAL u32x4*>(src + (size_t)l * I.Cp + o);
+        }
+        __syncthreads();
+    }
     // With the gradient, before each barrier a wavefront touches the record
     // of its first task of the next level, so that the record's loads
     // overlap the barrier (measured: faster with the pre-order pass, slower
@@ -628,7 +664,7 @@ __global__ void __launch_bounds__(512) k_fused(const ItemSched* __restrict__ ite
                 jn1 = lvl_a[I.la0 + 1];
             }
         }
-        for (int j = lvl_f[k] + w; j < j1; j += W) fwd_task<S, CPT>(jobs_f[j], r, tile, parts);
+        for (int j = lvl_f[k] + w; j < j1; j += W) fwd_task<S, CPT>(jobs_f[j], r, tile, parts, lrow);
         if (jn < jn1) warm_record(k + 1 < I.lf1 ? jobs_f[jn] : jobs_a[jn]);
         __syncthreads();
     }
@@ -637,7 +673,8 @@ __global__ void __launch_bounds__(512) k_fused(const ItemSched* __restrict__ ite
             const int j1 = lvl_a[k + 1];
             const int jn = j1 + w;
             const int jn1 = k + 1 < I.la1 ? lvl_a[k + 2] : 0;
-            for (int j = lvl_a[k] + w; j < j1; j += W) adj_task<S, CPT>(jobs_a[j], r, tile, parts);
+            for (int j = lvl_a[k] + w; j < j1; j += W)
+                adj_task<S, CPT>(jobs_a[j], r, tile, parts, nullptr, lrow);
             if (jn < jn1) warm_record(jobs_a[jn]);
             __syncthreads();
         }
@@ -806,18 +843,20 @@ hipError_t launch_level(bool adjoint, int S, int cpt, const LevelJob* jobs,
 }
 
 hipError_t launch_fused(bool grad, int S, int cpt, const ItemSched* items, int nitems,
-                        int nblocks, int waves, const LevelJob* jobs_f, const int32_t* lvl_f,
-                        const LevelJob* jobs_a, const int32_t* lvl_a, double* parts,
-                        hipStream_t st) {
+                        int nblocks, int waves, size_t lds_bytes, const LevelJob* jobs_f,
+                        const int32_t* lvl_f, const LevelJob* jobs_a, const int32_t* lvl_a,
+                        double* parts, hipStream_t st) {
     if (nitems <= 0 || nblocks <= 0) return hipSuccess;
     if (cpt == 0) cpt = 1;  // the fused schedule has no MFMA variant; same 64-pattern tile
 #define X(n, c)                                                                                  \
     if (S == n && cpt == c) {                                                                    \
         if (grad)                                                                                \
-            hipLaunchKernelGGL((k_fused<n, c, true>), dim3(nblocks), dim3(64 * waves), 0, st,    \
+            hipLaunchKernelGGL((k_fused<n, c, true>), dim3(nblocks), dim3(64 * waves), lds_bytes, \
+                               st,                                                               \
                                items, nitems, jobs_f, lvl_f, jobs_a, lvl_a, parts);              \
         else                                                                                     \
-            hipLaunchKernelGGL((k_fused<n, c, false>), dim3(nblocks), dim3(64 * waves), 0, st,   \
+            hipLaunchKernelGGL((k_fused<n, c, false>), dim3(nblocks), dim3(64 * waves),          \
+                               lds_bytes, st,                                                    \
                                items, nitems, jobs_f, lvl_f, jobs_a, lvl_a, parts);              \
         return hipGetLastError();                                                                \
     }
