@@ -260,8 +260,9 @@ int mcp_autotune(mcp_batch* batch, int32_t reps, double* report, int32_t cap, in
 
 /* Autotuner for the launch schedule: times LEVELS, HYBRID with widths 1, 2,
  * 4, 8, 16 and FUSED the same way and keeps the fastest; report rows are
- * (mode, hybrid width, ms).  mcp_get_schedule reads the choice (results are
- * bitwise identical across schedules). */
+ * (mode, hybrid width, ms).  mcp_get_schedule reads the choice (gradients
+ * are bitwise identical across schedules; logL agrees to rounding, because
+ * the fused launches sum a tile's log scalers as one product per lane). */
 int mcp_autotune_schedule(mcp_batch* batch, int32_t reps, double* report, int32_t cap,
                           int32_t* n_report);
 int mcp_get_schedule(mcp_batch* batch, int32_t* mode, int32_t* hybrid_width);

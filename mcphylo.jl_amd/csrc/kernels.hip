@@ -202,10 +202,32 @@ __device__ __forceinline__ void child_fetch(const ChildRef& ch, const LevelJob& 
     }
 }
 
-// post-order: partial of the job's node for the thread's columns
+// Per lane and column, the product of the log-scaler factors of every
+// post-order task a fused wavefront runs, kept as mant * 2^ex with mant in
+// [0.5, 1) (renormalised after every factor, so it neither underflows nor
+// overflows).  sum_v log(f_v) = log(mant) + ex log 2 replaces a log, a
+// weight load and a wavefront sum per task by one of each per pass.
+template <int CPT>
+struct ScaleProd {
+    double mant[CPT];
+    int ex[CPT];
+};
+
+template <int CPT>
+__device__ __forceinline__ void scale_fold(ScaleProd<CPT>& p, int k, double f) {
+    int e1, e2;
+    const double m = frexp(f, &e1);
+    p.mant[k] = frexp(p.mant[k] * m, &e2);
+    p.ex[k] += e1 + e2;
+}
+
+// post-order: partial of the job's node for the thread's columns.  Its
+// log-scaler sum goes to its own slot, or (prod != nullptr) is folded into
+// *prod and the slot gets 0.
 template <int S, int CPT>
 __device__ __forceinline__ void fwd_task(const LevelJob& J, int r, int tile,
-                                         double* __restrict__ parts, int lrow = 0) {
+                                         double* __restrict__ parts, int lrow = 0,
+                                         ScaleProd<CPT>* prod = nullptr) {
     const int lane = (int)(threadIdx.x & 63);
     const int c0 = tile * 64 * CPT + lane;
     // the record fields the task reads, fetched as one batch of scalar loads
@@ -248,8 +270,12 @@ __device__ __forceinline__ void fwd_task(const LevelJob& J, int r, int tile,
             double t = 0.0;
 #pragma unroll
             for (int s = 0; s < S; ++s) t = fma(acc[k][s], pi[s], t);
-            const double v = wts[c] * log(t);
-            contrib += c < J.C ? v : 0.0;
+            if (prod) {
+                scale_fold<CPT>(*prod, k, t);
+            } else {
+                const double v = wts[c] * log(t);
+                contrib += c < J.C ? v : 0.0;
+            }
         }
     } else {
         // by_max!: rescale by the state maximum, keep its log (VectorizedFunctions.jl:31-49)
@@ -264,9 +290,17 @@ __device__ __forceinline__ void fwd_task(const LevelJob& J, int r, int tile,
 #pragma unroll
             for (int s = 0; s < S; ++s) acc[k][s] *= inv;
             store_row<S>(dst + (size_t)c * S, acc[k]);
-            const double v = wts[c] * log(mx);
-            contrib += c < J.C ? v : 0.0;
+            if (prod) {
+                scale_fold<CPT>(*prod, k, mx);
+            } else {
+                const double v = wts[c] * log(mx);
+                contrib += c < J.C ? v : 0.0;
+            }
         }
+    }
+    if (prod) {
+        if (lane == 0) parts[J.lpart + r * J.ntiles + tile] = 0.0;
+        return;
     }
     const double bs = wave_sum(contrib);
     if (lane == 0) parts[J.lpart + r * J.ntiles + tile] = bs;
@@ -651,6 +685,14 @@ __global__ void __launch_bounds__(512) k_fused(const ItemSched* __restrict__ ite
     // of its first task of the next level, so that the record's loads
     // overlap the barrier (measured: faster with the pre-order pass, slower
     // for the post-order pass alone).
+    // the post-order tasks fold their log scalers into prod; the pass ends
+    // with one sum per block (see below)
+    ScaleProd<CPT> prod;
+#pragma unroll
+    for (int k = 0; k < CPT; ++k) {
+        prod.mant[k] = 1.0;
+        prod.ex[k] = 0;
+    }
     for (int k = I.lf0; k < I.lf1; ++k) {
         const int j1 = lvl_f[k + 1];
         // the next level's job range, read with this level's bounds
@@ -664,9 +706,35 @@ __global__ void __launch_bounds__(512) k_fused(const ItemSched* __restrict__ ite
                 jn1 = lvl_a[I.la0 + 1];
             }
         }
-        for (int j = lvl_f[k] + w; j < j1; j += W) fwd_task<S, CPT>(jobs_f[j], r, tile, parts, lrow);
+        for (int j = lvl_f[k] + w; j < j1; j += W)
+            fwd_task<S, CPT>(jobs_f[j], r, tile, parts, lrow, &prod);
         if (jn < jn1) warm_record(k + 1 < I.lf1 ? jobs_f[jn] : jobs_a[jn]);
         __syncthreads();
+    }
+    if (I.lf0 < I.lf1) {
+        // sum_c w_c log(prod_c) per wavefront, the wavefront sums added in a
+        // fixed order into the slot of the pass's first job (zeroed by
+        // wavefront 0's own task, earlier in its program order)
+        __shared__ double wsum[8];
+        const LevelJob& J0 = jobs_f[lvl_f[I.lf0]];
+        const int lane = (int)(threadIdx.x & 63);
+        const int c0 = tile * 64 * CPT + lane;
+        const MCP_GLOBAL double* wts = gptr(J0.weights);
+        double contrib = 0.0;
+#pragma unroll
+        for (int k = 0; k < CPT; ++k) {
+            const int c = c0 + 64 * k;
+            const double v = wts[c] * (log(prod.mant[k]) + (double)prod.ex[k] * M_LN2);
+            contrib += c < J0.C ? v : 0.0;
+        }
+        const double ws = wave_sum(contrib);
+        if (lane == 0) wsum[w] = ws;
+        __syncthreads();
+        if (w == 0 && lane == 0) {
+            double t = 0.0;
+            for (int i = 0; i < W; ++i) t += wsum[i];
+            parts[J0.lpart + r * J0.ntiles + tile] = t;
+        }
     }
     if constexpr (GRAD) {
         for (int k = I.la0; k < I.la1; ++k) {

@@ -225,10 +225,11 @@ def test_deterministic_and_schedules_agree():
     a1 = p.logl_grad(b)
     a2 = p.logl_grad(b)
     assert a1[0] == a2[0] and np.array_equal(a1[1], a2[1])
-    # the two schedules run the same tasks and the same reductions
+    # the two schedules run the same tasks; the gradient reductions are the
+    # same, the fused log-scaler sum is one product per lane (rounding only)
     p.set_schedule("levels")
     a3 = p.logl_grad(b)
-    assert a3[0] == a1[0] and np.array_equal(a3[1], a1[1])
+    assert rel_err(a3[0], a1[0]) < 1e-13 and np.array_equal(a3[1], a1[1])
 
 
 def test_errors():

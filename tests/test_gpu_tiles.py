@@ -76,7 +76,7 @@ def test_autotune_mixed_batch_keeps_results_and_reports_candidates():
 
 
 @pytest.mark.parametrize("width", [1, 2, 4, 8])
-def test_hybrid_schedule_is_bitwise_levels(width):
+def test_hybrid_and_fused_schedules_match_levels(width):
     data = mcp.synthetic.make_batched_workload(n_trees=5, chains=2, n_range=(16, 80),
                                                c_range=(300, 3000))
     ds, items, models, blv = data
@@ -100,9 +100,13 @@ def test_hybrid_schedule_is_bitwise_levels(width):
         res[mode] = (ll, g, l_only)
         b.close()
     for mode in ("hybrid", "fused"):
-        assert np.array_equal(res[mode][0], res["levels"][0]), mode
+        # A fused launch folds each tile's log scalers into one product per
+        # lane (one log per pass instead of one per node), so logL agrees
+        # with the levels schedule to rounding.  The gradient sums are the
+        # same operations in the same order: bit for bit.
+        np.testing.assert_allclose(res[mode][0], res["levels"][0], rtol=1e-13, atol=0)
         assert np.array_equal(res[mode][1], res["levels"][1]), mode
-        assert np.array_equal(res[mode][2], res["levels"][2]), mode
+        np.testing.assert_allclose(res[mode][2], res["levels"][2], rtol=1e-13, atol=0)
 
 
 def test_hybrid_schedule_single_tree_vs_oracle_and_schedule_autotune():
