@@ -213,6 +213,15 @@ struct ScaleProd {
     int ex[CPT];
 };
 
+// Partials are rescaled by 2^-e, e the binary exponent of their state
+// maximum (the maximum lands in [0.5, 1)): the reference's by_max! divides
+// by the maximum itself.  Both keep the partial in range; the power of two
+// is exact and needs no division, and its log-scaler term is e log 2
+// instead of log(max).  Every log-scaler, ratio and gradient is invariant
+// to the choice.  frexp_exp is 0 for 0, inf and NaN, so impossible
+// columns still give -inf at the root and NaN still propagates.
+__device__ __forceinline__ int scale_exp(double mx) { return __builtin_amdgcn_frexp_exp(mx); }
+
 template <int CPT>
 __device__ __forceinline__ void scale_fold(ScaleProd<CPT>& p, int k, double f) {
     int e1, e2;
@@ -278,7 +287,8 @@ __device__ __forceinline__ void fwd_task(const LevelJob& J, int r, int tile,
             }
         }
     } else {
-        // by_max!: rescale by the state maximum, keep its log (VectorizedFunctions.jl:31-49)
+        // by_max! (VectorizedFunctions.jl:31-49), rescaling by the power of
+        // two at the state maximum; keep its log
         MCP_GLOBAL double* dst = gptr_w(J.F) + (size_t)r * J.Cp * S;
 #pragma unroll
         for (int k = 0; k < CPT; ++k) {
@@ -286,14 +296,14 @@ __device__ __forceinline__ void fwd_task(const LevelJob& J, int r, int tile,
             double mx = -INFINITY;
 #pragma unroll
             for (int s = 0; s < S; ++s) mx = (mx < acc[k][s]) ? acc[k][s] : mx;
-            const double inv = 1.0 / mx;
+            const int e = scale_exp(mx);
 #pragma unroll
-            for (int s = 0; s < S; ++s) acc[k][s] *= inv;
+            for (int s = 0; s < S; ++s) acc[k][s] = ldexp(acc[k][s], -e);
             store_row<S>(dst + (size_t)c * S, acc[k]);
             if (prod) {
-                scale_fold<CPT>(*prod, k, mx);
+                prod->ex[k] += e;
             } else {
-                const double v = wts[c] * log(mx);
+                const double v = wts[c] * ((double)e * M_LN2);
                 contrib += c < J.C ? v : 0.0;
             }
         }
@@ -311,7 +321,8 @@ __device__ __forceinline__ void fwd_task(const LevelJob& J, int r, int tile,
 //   g_u = sum_m T_u[m] (dP_u f_u)[m]                        (:39-48)
 //   G_u[k] = sum_m P_u[m][k] T_u[m]                         (:50-57)
 //   col = sum_k f_u[k] G_u[k];  grad_u += w_c g_u / col     (:59-69)
-//   G_u /= max_k G_u[k] (internal u)                        (:71-73)
+//   G_u /= max_k G_u[k] (internal u; here the power of two   (:71-73)
+//   at the maximum, see scale_exp)
 // (LikelihoodCalculator_Node.jl).  For a leaf u, g_u and col are dot
 // products of T_u with the leaf's dP.tip and P.tip rows.
 template <int S, int CPT>
@@ -368,9 +379,10 @@ __device__ __forceinline__ double child_adjoint(const ChildRef& ch, const LevelJ
                 col = fma(f[k][s], g[s], col);
                 mx = (mx < g[s]) ? g[s] : mx;
             }
-            const double inv = 1.0 / mx;
+            // rescale by the power of two at the maximum (see scale_exp)
+            const int e = scale_exp(mx);
 #pragma unroll
-            for (int s = 0; s < S; ++s) g[s] *= inv;
+            for (int s = 0; s < S; ++s) g[s] = ldexp(g[s], -e);
             store_row<S>(Gdst + (size_t)c * S, g);
             colv[k] = col;
             const double v = wts[c] * (num / col);
