@@ -202,6 +202,31 @@ __device__ __forceinline__ void child_fetch(const ChildRef& ch, const LevelJob& 
     }
 }
 
+// Pre-order form of child_fetch: a leaf child also gets its dP.tip rows
+// dPT[r][code] (dm), loaded beside its P.tip rows rather than after the
+// task's arithmetic.  Used where the registers allow it (S * CPT <= 8).
+template <int S, int CPT>
+constexpr bool kLeafRowsEarly = S * CPT <= 8;
+
+template <int S, int CPT>
+__device__ __forceinline__ void child_fetch_pre(const ChildRef& ch, const LevelJob& J, int r,
+                                                int c0, int lrow, double (&f)[CPT][S],
+                                                double (&m)[CPT][S], double (&dm)[CPT][S]) {
+    if (ch.codes) {
+        const MCP_GLOBAL double* pt = gptr(ch.P) + (size_t)r * J.K * S;
+        const MCP_GLOBAL double* dpt = gptr(ch.dP) + (size_t)r * J.K * S;
+        int code[CPT];
+        leaf_codes<CPT>(ch, c0, lrow, code);
+#pragma unroll
+        for (int k = 0; k < CPT; ++k) {
+            select_row<S>(pt, code[k], m[k]);
+            select_row<S>(dpt, code[k], dm[k]);
+        }
+    } else {
+        child_fetch<S, CPT>(ch, J, r, c0, lrow, f, m);
+    }
+}
+
 // Per lane and column, the product of the log-scaler factors of every
 // post-order task a fused wavefront runs, kept as mant * 2^ex with mant in
 // [0.5, 1) (renormalised after every factor, so it neither underflows nor
@@ -325,23 +350,29 @@ __device__ __forceinline__ void fwd_task(const LevelJob& J, int r, int tile,
 //   at the maximum, see scale_exp)
 // (LikelihoodCalculator_Node.jl).  For a leaf u, g_u and col are dot
 // products of T_u with the leaf's dP.tip and P.tip rows.
-template <int S, int CPT>
+template <int S, int CPT, bool DM = false>
 __device__ __forceinline__ double child_adjoint(const ChildRef& ch, const LevelJob& J, int r, int c0,
                                                 const double (&T)[CPT][S],
                                                 const double (&f)[CPT][S],
                                                 const double (&m)[CPT][S],
-                                                double (&colv)[CPT], int lrow) {
+                                                double (&colv)[CPT], int lrow,
+                                                const double (&dm)[CPT][S]) {
     const MCP_GLOBAL double* wts = gptr(J.weights);
     double contrib = 0.0;
     if (ch.codes) {
         const MCP_GLOBAL double* dpt = gptr(ch.dP) + (size_t)r * J.K * S;
         int code[CPT];
-        leaf_codes<CPT>(ch, c0, lrow, code);
+        if constexpr (!DM) leaf_codes<CPT>(ch, c0, lrow, code);
 #pragma unroll
         for (int k = 0; k < CPT; ++k) {
             const int c = c0 + 64 * k;
             double row[S];
-            select_row<S>(dpt, code[k], row);
+            if constexpr (DM) {
+#pragma unroll
+                for (int s = 0; s < S; ++s) row[s] = dm[k][s];
+            } else {
+                select_row<S>(dpt, code[k], row);
+            }
             double num = 0.0, col = 0.0;
 #pragma unroll
             for (int s = 0; s < S; ++s) {
@@ -495,20 +526,30 @@ __device__ __forceinline__ void adj_task(const LevelJob& J, int r, int tile,
     }
     if (J.nchild == 2) {
         double fa[CPT][S], ma[CPT][S], fb[CPT][S], mb[CPT][S], T[CPT][S];
-        child_fetch<S, CPT>(a, J, r, c0, lrow, fa, ma);
-        child_fetch<S, CPT>(b, J, r, c0, lrow, fb, mb);
+        constexpr bool E = kLeafRowsEarly<S, CPT>;
+        double da[E ? CPT : 1][S], db[E ? CPT : 1][S];
+        if constexpr (E) {
+            child_fetch_pre<S, CPT>(a, J, r, c0, lrow, fa, ma, da);
+            child_fetch_pre<S, CPT>(b, J, r, c0, lrow, fb, mb, db);
+        } else {
+            child_fetch<S, CPT>(a, J, r, c0, lrow, fa, ma);
+            child_fetch<S, CPT>(b, J, r, c0, lrow, fb, mb);
+        }
 #pragma unroll
         for (int k = 0; k < CPT; ++k)
 #pragma unroll
             for (int s = 0; s < S; ++s) T[k][s] = gp[k][s] * mb[k][s];
         double colv[CPT];
-        const double ca = child_adjoint<S, CPT>(a, J, r, c0, T, fa, ma, colv, lrow);
+        double ca, cb;
+        if constexpr (E) ca = child_adjoint<S, CPT, true>(a, J, r, c0, T, fa, ma, colv, lrow, da);
+        else ca = child_adjoint<S, CPT>(a, J, r, c0, T, fa, ma, colv, lrow, ma);
         if constexpr (MG) mg_branch<S, CPT>(a, J, r, tile, c0, T, fa, colv, mparts);
 #pragma unroll
         for (int k = 0; k < CPT; ++k)
 #pragma unroll
             for (int s = 0; s < S; ++s) T[k][s] = gp[k][s] * ma[k][s];
-        const double cb = child_adjoint<S, CPT>(b, J, r, c0, T, fb, mb, colv, lrow);
+        if constexpr (E) cb = child_adjoint<S, CPT, true>(b, J, r, c0, T, fb, mb, colv, lrow, db);
+        else cb = child_adjoint<S, CPT>(b, J, r, c0, T, fb, mb, colv, lrow, mb);
         if constexpr (MG) {
             mg_branch<S, CPT>(b, J, r, tile, c0, T, fb, colv, mparts);
             if (!J.G) {
@@ -546,7 +587,7 @@ __device__ __forceinline__ void adj_task(const LevelJob& J, int r, int tile,
         }
         child_fetch<S, CPT>(u, J, r, c0, lrow, fu, mu_);
         double colv[CPT];
-        const double cu = child_adjoint<S, CPT>(u, J, r, c0, T, fu, mu_, colv, lrow);
+        const double cu = child_adjoint<S, CPT>(u, J, r, c0, T, fu, mu_, colv, lrow, mu_);
         if constexpr (MG) mg_branch<S, CPT>(u, J, r, tile, c0, T, fu, colv, mparts);
         const double su = wave_sum(cu);
         if (lane == 0) parts[u.gpart + r * J.ntiles + tile] = su;
