@@ -47,6 +47,20 @@ __device__ __forceinline__ void afrag20(const MCP_GLOBAL double* __restrict__ M,
         }
 }
 
+// The same fragments of a matrix staged in LDS (row-major, 20 x 20).
+__device__ __forceinline__ void afrag20_lds(const double* M, bool transpose, int lane,
+                                            double (&a)[10]) {
+    const int i = lane & 15, g = lane >> 4;
+#pragma unroll
+    for (int mb = 0; mb < 2; ++mb)
+#pragma unroll
+        for (int kc = 0; kc < 5; ++kc) {
+            const int row = 16 * mb + i, col = 4 * kc + g;
+            a[5 * mb + kc] = row < kS20 ? (transpose ? M[col * kS20 + row] : M[row * kS20 + col])
+                                        : 0.0;
+        }
+}
+
 // y[t] = row g + 4t of (M . X) for the lane's column, from B fragments
 // xb[t] = X[4t + g][column].
 __device__ __forceinline__ void mm20(const double (&a)[10], const double (&xb)[5], double (&y)[5]) {
@@ -207,31 +221,57 @@ __device__ __forceinline__ void gp_load20(const LevelJob& J, int r, int c, int g
     }
 }
 
-// pre-order task: 64 columns of one (parent, category)
+// The two children's P and dP of one (parent, category) staged in LDS by
+// k_level20 (smat: Pa, dPa, Pb, dPb, 400 doubles each; a leaf child's
+// slots are unused).  The pre-order task needs P, dP and P^T fragments of
+// both children for each of its four 16-column subtiles; the registers do
+// not hold them across subtiles (two waves per SIMD), and re-reading them
+// from LDS replaces about 60 scattered global loads per subtile.
+constexpr int kStageDoubles = 4 * kS20 * kS20;
+
+__device__ __forceinline__ void stage_mats20(const LevelJob& J, int r, double* smat) {
+    const int lane = (int)(threadIdx.x & 63);
+    const ChildRef& a = J.kid[0];
+    const ChildRef& b = J.kid[1];
+    const size_t off = (size_t)r * kS20 * kS20;
+    for (int i = lane; i < kS20 * kS20; i += 64) {
+        if (!a.codes) {
+            smat[i] = gptr(a.P)[off + i];
+            smat[kS20 * kS20 + i] = gptr(a.dP)[off + i];
+        }
+        if (!b.codes) {
+            smat[2 * kS20 * kS20 + i] = gptr(b.P)[off + i];
+            smat[3 * kS20 * kS20 + i] = gptr(b.dP)[off + i];
+        }
+    }
+}
+
+// pre-order task: 64 columns of one (parent, category); smat holds the
+// children's matrices for (J, r) when J has two children
 __device__ __forceinline__ void adj_task20(const LevelJob& J, int r, int tile,
-                                           double* __restrict__ parts) {
+                                           double* __restrict__ parts, const double* smat) {
     const int lane = (int)(threadIdx.x & 63), i = lane & 15, g = lane >> 4;
     const MCP_GLOBAL double* wts = gptr(J.weights);
     if (J.nchild == 2) {
         const ChildRef& a = J.kid[0];
         const ChildRef& b = J.kid[1];
-        const MCP_GLOBAL double* Pa = gptr(a.P) + (size_t)r * kS20 * kS20;
-        const MCP_GLOBAL double* Pb = gptr(b.P) + (size_t)r * kS20 * kS20;
-        const MCP_GLOBAL double* dPa = gptr(a.dP) + (size_t)r * kS20 * kS20;
-        const MCP_GLOBAL double* dPb = gptr(b.dP) + (size_t)r * kS20 * kS20;
+        const double* Pa = smat;
+        const double* dPa = smat + kS20 * kS20;
+        const double* Pb = smat + 2 * kS20 * kS20;
+        const double* dPb = smat + 3 * kS20 * kS20;
         double ca = 0.0, cb = 0.0;
 #pragma unroll 1
         for (int st = 0; st < 4; ++st) {
-            // matrix fragments are re-read per tile (they stay in L1) to keep
-            // the live register set small
+            // matrix fragments are re-read per subtile from LDS, to keep the
+            // live register set small
             const int c = tile * 64 + 16 * st + i;
             double gp[5], fa[5], ma[5], fb[5], mb[5], T[5];
             gp_load20(J, r, c, g, gp);
             {
                 double fr[10];
-                if (!a.codes) afrag20(Pa, false, lane, fr);
+                if (!a.codes) afrag20_lds(Pa, false, lane, fr);
                 msg20(a, J, r, c, g, fr, fa, ma);
-                if (!b.codes) afrag20(Pb, false, lane, fr);
+                if (!b.codes) afrag20_lds(Pb, false, lane, fr);
                 msg20(b, J, r, c, g, fr, fb, mb);
             }
             const bool own = g == 0 && c < J.C;
@@ -239,15 +279,15 @@ __device__ __forceinline__ void adj_task20(const LevelJob& J, int r, int tile,
 #pragma unroll
             for (int t = 0; t < 5; ++t) T[t] = gp[t] * mb[t];
             if (!a.codes) {
-                afrag20(dPa, false, lane, dfr);
-                afrag20(Pa, true, lane, tfr);
+                afrag20_lds(dPa, false, lane, dfr);
+                afrag20_lds(Pa, true, lane, tfr);
             }
             const double va = wts[c] * child_adj20(a, J, r, c, g, T, fa, ma, dfr, tfr);
 #pragma unroll
             for (int t = 0; t < 5; ++t) T[t] = gp[t] * ma[t];
             if (!b.codes) {
-                afrag20(dPb, false, lane, dfr);
-                afrag20(Pb, true, lane, tfr);
+                afrag20_lds(dPb, false, lane, dfr);
+                afrag20_lds(Pb, true, lane, tfr);
             }
             const double vb = wts[c] * child_adj20(b, J, r, c, g, T, fb, mb, dfr, tfr);
             ca += own ? va : 0.0;
@@ -305,6 +345,8 @@ __attribute__((amdgpu_waves_per_eu(ADJ ? MCP_ADJ20_WAVES_PER_EU : 1))) k_level20
     const int t1 = min(total, t0 + chunk);
     int j = blk_job[blockIdx.x];
     int jbeg = task_off[j], jend = task_off[j + 1];
+    __shared__ double smat[ADJ ? kStageDoubles : 1];
+    int sj = -1, sr = -1;  // (job, category) whose matrices smat holds
     for (int t = t0; t < t1; ++t) {
         while (t >= jend) {
             ++j;
@@ -315,8 +357,18 @@ __attribute__((amdgpu_waves_per_eu(ADJ ? MCP_ADJ20_WAVES_PER_EU : 1))) k_level20
         const int local = t - jbeg;
         const int r = uniform(local / J.ntiles);
         const int tile = uniform(local - r * J.ntiles);
-        if constexpr (ADJ) adj_task20(J, r, tile, parts);
-        else fwd_task20(J, r, tile, parts);
+        if constexpr (ADJ) {
+            if (J.nchild == 2 && (j != sj || r != sr)) {
+                __syncthreads();  // the previous task's fragment reads are done
+                stage_mats20(J, r, smat);
+                __syncthreads();
+                sj = j;
+                sr = r;
+            }
+            adj_task20(J, r, tile, parts, smat);
+        } else {
+            fwd_task20(J, r, tile, parts);
+        }
     }
 }
 
