@@ -19,7 +19,7 @@ fallback.
 from ._lib import McpArgumentError, McpDimensionMismatch, McpError, build, load_library
 from .engine import Batch, Model, Plan
 from .models import GTR, JC, ComplexEigensystemError, Restriction, freeK, setmatrix
-from .nexus import ParseNexus, compress_patterns, datafortree
+from .nexus import ParseCSV, ParseNexus, compress_patterns, create_csvdf, datafortree
 from .phylodist import (MultiplePhyloDist, PhyloDist, __logpdf, clear_plan_cache, gradlogpdf,
                         gradlogpdf_params, logpdf, maximum, minimum, size)
 from .rates import discrete_gamma_rates, mean_boundaries, median_boundaries
@@ -33,7 +33,7 @@ __all__ = [
     "build", "load_library", "McpError", "McpArgumentError", "McpDimensionMismatch",
     "Plan", "Batch", "Model", "Surrogate",
     "JC", "GTR", "Restriction", "freeK", "setmatrix", "ComplexEigensystemError",
-    "ParseNexus", "datafortree", "compress_patterns",
+    "ParseNexus", "ParseCSV", "create_csvdf", "datafortree", "compress_patterns",
     "PhyloDist", "MultiplePhyloDist", "logpdf", "gradlogpdf", "gradlogpdf_params", "__logpdf",
     "clear_plan_cache",
     "size", "minimum", "maximum",

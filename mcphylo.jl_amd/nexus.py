@@ -1,4 +1,5 @@
-"""Alignment ingestion: the NEXUS subset the reference reads and `datafortree`.
+"""Alignment ingestion: the NEXUS subset and the CSV format the reference
+reads, and `datafortree`.
 
 `ParseNexus` follows `src/Parser/ParseNexus.jl:10-130` (MrBayes-style,
 non-interleaved DATA block; `symbols` from the FORMAT line, otherwise the
@@ -18,7 +19,7 @@ import numpy as np
 
 from .tree import Node, post_order
 
-__all__ = ["ParseNexus", "datafortree", "compress_patterns"]
+__all__ = ["ParseNexus", "ParseCSV", "create_csvdf", "datafortree", "compress_patterns"]
 
 
 def ParseNexus(filename: str):
@@ -76,6 +77,37 @@ def ParseNexus(filename: str):
     else:
         alph = [ch for ch in symbols]
     return ntax, nchar, gap, missing, alph, df, langs
+
+
+def create_csvdf(filecontent, separator: str = ","):
+    """(langs, df) from lines `name<sep>c1<sep>c2...` (`src/Parser/ParseCSV.jl:44-62`):
+    the first field of a line is the taxon name, and df holds the first
+    character of every other field, one row per line."""
+    langs: list[str] = []
+    rows: list[list[str]] = []
+    for line in filecontent:
+        fields = line.split(separator)
+        langs.append(fields[0])
+        rows.append([f[0] for f in fields[1:]])
+    width = len(rows[-1]) if rows else 0
+    if any(len(r) != width for r in rows):
+        raise ValueError("rows of the matrix have different lengths")
+    return langs, np.array(rows, dtype="<U1").reshape(len(rows), width)
+
+
+def ParseCSV(filename: str, gap: str, miss: str, header: bool = True):
+    """MrBayes-convention CSV alignment (`src/Parser/ParseCSV.jl:17-36`).
+    Returns (ntax, nchar, gap, miss, symbols, df, langs) like ParseNexus;
+    `header` skips the first line; symbols are the sorted alphabet of the
+    matrix without the gap and missing symbols (`ParseNexus.jl:38-49`)."""
+    with open(filename, "r") as fh:
+        content = fh.read().splitlines()
+    if header and content:
+        content.pop(0)
+    langs, df = create_csvdf(content)
+    ntax, nchar = df.shape
+    alph = sorted({ch for ch in df.ravel() if ch != gap and ch != miss})
+    return ntax, nchar, gap, miss, alph, df, langs
 
 
 def datafortree(df, leave_names, tree: Node, symbols, gap: str, miss: str,

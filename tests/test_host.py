@@ -114,6 +114,49 @@ def test_parse_nexus_fixtures():
     assert (ntax, nchar) == (10, 100)
 
 
+# Expected values of the reference's own parser tests (test/parser/csv.jl,
+# test/parser/parser.jl) on its fixtures example.csv / example2.nex.
+CSV_MATRIX = ["00000-00001100-00?00?00", "0-0000?0000??00?0001??0", "0010000000-000100?00000"]
+CSV_LANGS = ["Swedish_0", "Welsh_N_0", "Sardinian_N_0"]
+
+
+def test_parse_csv_matches_reference_tests():
+    ntax, nchar, gap, miss, sym, df, langs = mcp.ParseCSV(os.path.join(GOLDEN, "example.csv"),
+                                                          "-", "?", False)
+    assert (ntax, nchar, gap, miss) == (3, 23, "-", "?")
+    assert list(sym) == ["0", "1"] and langs == CSV_LANGS
+    assert ["".join(r) for r in df] == CSV_MATRIX
+    content = [name + "," + ",".join(row) for name, row in zip(CSV_LANGS, CSV_MATRIX)]
+    l1, d1 = mcp.create_csvdf(content)
+    l2, d2 = mcp.create_csvdf([c.replace(",", ":") for c in content], ":")
+    assert l1 == l2 == CSV_LANGS and np.array_equal(d1, df) and np.array_equal(d2, df)
+    # a header line is skipped
+    with open(os.path.join(GOLDEN, "example.csv")) as fh:
+        first = fh.readline().strip()
+    assert mcp.ParseCSV(os.path.join(GOLDEN, "example.csv"), "-", "?", True)[6] == CSV_LANGS[1:]
+    assert first.startswith(CSV_LANGS[0])
+    # the same alignment as Nexus parses to the same tuple
+    nx = mcp.ParseNexus(os.path.join(GOLDEN, "example2.nex"))
+    assert nx[:5] == (3, 23, "-", "?", ["0", "1"]) and nx[6] == CSV_LANGS
+    assert np.array_equal(nx[5], df)
+
+
+def test_datafortree_on_csv_matches_reference_tip_columns():
+    # make_tree_with_data's expected one-hot columns of nodes 1 and 2
+    # (Sardinian_N_0 and Welsh_N_0), gaps and missing as all-ones
+    dim1 = np.array([[1, 1, 0, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 0, 1, 1, 1, 1, 1, 1, 1, 1],
+                     [0, 0, 1, 0, 0, 0, 0, 0, 0, 0, 1, 0, 0, 0, 1, 0, 0, 1, 0, 0, 0, 0, 0]], float)
+    dim2 = np.array([[1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 0, 1, 1, 1],
+                     [0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 1, 0, 0, 1, 0, 0, 0, 1, 1, 1, 0]], float)
+    ntax, nchar, gap, miss, sym, df, langs = mcp.ParseCSV(os.path.join(GOLDEN, "example.csv"),
+                                                          "-", "?", False)
+    t = mcp.ParseNewick("((Sardinian_N_0:0.1,Welsh_N_0:0.2):0.1,Swedish_0:0.3);")
+    x = mcp.datafortree(df, langs, t, sym, gap, miss)
+    by_name = {v.name: v.num for v in mcp.post_order(t)}
+    assert by_name["Sardinian_N_0"] == 1 and by_name["Welsh_N_0"] == 2
+    assert np.array_equal(x[:, :, 0], dim1) and np.array_equal(x[:, :, 1], dim2)
+
+
 def test_newick_numbering_and_blv():
     t = mcp.ParseNewick(SIMU_NEWICK)
     po = mcp.post_order(t)
