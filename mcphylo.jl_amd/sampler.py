@@ -10,7 +10,8 @@ The sampler restates the tree sampler of MCPhylo for a fixed topology:
   refraction! (src/samplers/tree_hamiltonian/refraction.jl:2-91): the
       leapfrog with time-ordered reflection at zero, molifier and scale_fac
       (auxilliary.jl:113-119) -- executed on the device, without the NNI
-      topology attempt;
+      topology attempt; with it, ref_NNI! (refraction.jl:35-91) in
+      NNIRefractionLeapfrog (host loop, logL from the engine);
   nutsepsilon (auxilliary.jl:6-51): initial step size;
   dual_averaging / setadapt! (pnuts.jl:145-176): step-size adaptation with
       (gamma, kappa, t0) = (0.05, 0.75, 10) and mu = log(10 eps);
@@ -37,7 +38,8 @@ import numpy as np
 
 __all__ = ["HState", "NUTSChain", "run_nuts", "ess_bm", "mcse_bm", "EngineLeapfrog",
            "HostLeapfrog", "SurrogateLeapfrog", "run_da_nuts", "exponential_logprior",
-           "NativeNUTS", "run_da_nuts_native"]
+           "NativeNUTS", "run_da_nuts_native", "TreePosition", "nni_parent",
+           "NNIRefractionLeapfrog"]
 
 
 class HState:
@@ -90,7 +92,9 @@ class NUTSChain:
         is not reversible there.  molifier=False sets delta to 1e-300: pure
         reflection at zero, an exact sampler of the posterior."""
         self.molifier = bool(molifier)
-        self.x = np.array(x0, dtype=np.float64)
+        # subok: a TreePosition keeps its topology
+        self.x = np.array(x0, dtype=np.float64, subok=True)
+        self.topologies: list[np.ndarray] = []  # per draw, for TreePosition chains
         self.rng = rng
         self.target = float(target)
         self.tree_depth = int(tree_depth)
@@ -349,6 +353,145 @@ def _host_refraction(x, r, g, eps, delta, max_reflect=None):
     return xm, r
 
 
+# ---------------------------------------------------------------------------
+# refraction with NNI moves (refraction.jl:2-91)
+# ---------------------------------------------------------------------------
+
+class TreePosition(np.ndarray):
+    """Branch lengths by node num (blv) that carry their topology: `parent`
+    (int32 [N], 1-based mother num, 0 at the root num N).  Copies and
+    arithmetic keep the attribute, so a NUTS chain moves tree positions
+    through its trajectories unchanged (the reference's state holds the
+    tree itself)."""
+
+    def __new__(cls, blv, parent):
+        obj = np.array(blv, dtype=np.float64).view(cls)
+        obj.parent = np.array(parent, dtype=np.int32)
+        return obj
+
+    def __array_finalize__(self, obj):
+        self.parent = getattr(obj, "parent", None)
+
+
+def nni_parent(parent, v: int, pick: int):
+    """Nearest-neighbour interchange across the branch above node num v:
+    child `pick` (0 or 1, children in ascending num) of v trades places with
+    v's sibling.  Returns (new parent array, 1), or (parent, 0) when v is a
+    leaf or the root or v or its mother is not bifurcating.  Node nums and
+    so the branch-length vector are unchanged.
+
+    The reference calls MCPhyloTree's NNI!(tree, index) (refraction.jl:65),
+    a dependency this repository does not hold, so the choice of swap is a
+    restatement (parity unpinned): one of the two interchanges around the
+    branch, picked by the caller."""
+    par = np.asarray(parent, dtype=np.int32)
+    N = par.shape[0]
+    if v < 1 or v >= N:
+        return par, 0
+    kids = np.flatnonzero(par == v) + 1
+    p = int(par[v - 1])
+    sibs = [int(u) for u in np.flatnonzero(par == p) + 1 if u != v]
+    if kids.shape[0] != 2 or len(sibs) != 1:
+        return par, 0
+    c = int(kids[pick & 1])
+    new = par.copy()
+    new[c - 1] = p
+    new[sibs[0] - 1] = v
+    return new, 1
+
+
+def _molify(y, delta):
+    return np.where(y >= delta, y, (y * y + delta * delta) / (2.0 * delta))
+
+
+class NNIRefractionLeapfrog:
+    """refraction! with NNI attempts (refraction.jl:2-91) on one engine.
+
+    Positions are TreePosition.  A branch whose end is internal and that
+    reaches zero during the step triggers ref_NNI!'s attempt: the log
+    posterior of the current tree (at the molified reflection point) and of
+    the interchanged tree are compared, and the interchange is kept when the
+    branch's squared momentum exceeds 2 (U_before - U_after), the momentum
+    then shrinking to sqrt(r^2 - 2 dU).  logL comes from the engine (its
+    topology is switched with set_topology); prior(x) -> (logprior, grad)
+    adds the tree prior.  Counts nni (made) and att_nni (attempted) as the
+    reference does.  Multiple chains share the engine one request at a
+    time."""
+
+    def __init__(self, plan, prior=None, rng=None, max_reflect: int | None = None):
+        self.plan = plan
+        self.prior = prior
+        self.rng = rng if rng is not None else np.random.default_rng(0)
+        self.max_reflect = max_reflect
+        self.nni = 0
+        self.att_nni = 0
+
+    def _use(self, parent):
+        if not np.array_equal(self.plan.parent, parent):
+            self.plan.set_topology(parent)
+
+    def _logpost(self, x):
+        lf = self.plan.logl(np.asarray(x, dtype=np.float64))
+        if self.prior is not None:
+            lf += self.prior(np.asarray(x))[0]
+        return lf
+
+    def one(self, x, r, g, eps, delta):
+        topo = np.array(x.parent, dtype=np.int32)
+        N = topo.shape[0]
+        internal = np.zeros(N - 1, dtype=bool)
+        for u in topo[:N - 1]:
+            if 1 <= u <= N - 1:
+                internal[u - 1] = True
+        blv = np.asarray(x, dtype=np.float64).copy()
+        r = np.asarray(r, dtype=np.float64) + (eps * 0.5) * np.asarray(g)
+        tmp = blv + eps * r
+        t = 0.0
+        cap = 64 * max(1, blv.shape[0]) if self.max_reflect is None else self.max_reflect
+        for _ in range(cap):
+            if not np.any(tmp <= 0.0):
+                break
+            with np.errstate(divide="ignore", invalid="ignore"):
+                tl = tmp / np.abs(r)
+            k = int(np.argmin(tl))
+            temp = eps - t + tl[k]
+            blv = np.abs(blv + temp * r)
+            r = r.copy()
+            r[k] = -r[k]
+            if internal[k]:
+                xm = _molify(blv, delta)
+                self._use(topo)
+                u_before = self._logpost(xm)
+                cand, made = nni_parent(topo, k + 1, int(self.rng.integers(2)))
+                if made:
+                    self.att_nni += 1
+                    self._use(cand)
+                    u_after = self._logpost(xm)
+                    d_u = 2.0 * (u_before - u_after)
+                    v2 = r[k] * r[k]
+                    if v2 > d_u:
+                        self.nni += made
+                        # positive whatever the direction of integration, as
+                        # in the reference (refraction.jl:76)
+                        r[k] = math.sqrt(v2 - d_u)
+                        topo = cand
+            t = eps + tl[k]
+            tmp = blv + (eps - t) * r
+        xm = _molify(tmp, delta)
+        self._use(topo)
+        lf, gr = self.plan.logl_grad(xm)
+        if self.prior is not None:
+            lpr, gpr = self.prior(xm)
+            lf += lpr
+            gr = gr + gpr
+        gr = gr * np.where(xm < delta, xm / delta, 1.0)
+        r = r + (eps * 0.5) * gr
+        return HState(TreePosition(xm, topo), r, gr, float(lf))
+
+    def __call__(self, reqs):
+        return [self.one(*q) for q in reqs]
+
+
 def run_nuts(chains, leapfrog, n_iter: int, n_adapt: int = 0, trace=None):
     """Advance every chain through n_iter transitions (the first n_adapt with
     step-size adaptation) in lockstep: each round issues one batched
@@ -383,6 +526,8 @@ def run_nuts(chains, leapfrog, n_iter: int, n_adapt: int = 0, trace=None):
                 if done_iters[i] >= n_adapt:
                     draws[i].append(chains[i].x.copy())
                     lfs[i].append(chains[i].lf)
+                    if getattr(chains[i].x, "parent", None) is not None:
+                        chains[i].topologies.append(chains[i].x.parent.copy())
                 done_iters[i] += 1
                 if done_iters[i] < n_iter + n_adapt:
                     start(i)

@@ -222,3 +222,60 @@ def test_tree_length_posterior_of_the_reference_pnuts_test():
     print(f"tree length posterior {m:.3f} +- {sd:.3f}", flush=True)
     assert m - sd <= 1.8 <= m + sd, (m, sd)
     assert np.all(np.isfinite(lp))
+
+
+def test_nni_refraction_on_the_engine_matches_the_oracle_and_samples():
+    """refraction! with NNI attempts (refraction.jl:35-91) with the engine's
+    logL: the same steps as with the CPU oracle (same RNG, so the same
+    interchange choices and acceptances), and a short NUTS run that moves
+    the topology and keeps every state consistent with its tree."""
+    w = mcp.synthetic.make_workload("custom", seed=631, n=12, S=4, C=300, R=1)
+    par0 = np.array(w.ta.parent, dtype=np.int32)
+    N = len(par0)
+    blv = mcp.get_branchlength_vector(w.tree).copy()
+    internal = [v for v in range(1, N) if np.count_nonzero(par0 == v) == 2]
+    for v in internal[:3]:
+        blv[v - 1] = 2e-3                      # short internal branches reach zero
+    plan = mcp.Plan(w.ta.parent, w.x, 1)
+    plan.set_model(mcp.Model(w.U, w.D, w.Uinv, w.mu, w.rates, w.pi))
+
+    class OraclePlan:
+        def __init__(self):
+            self.parent = par0.copy()
+
+        def set_topology(self, parent):
+            self.parent = np.array(parent, dtype=np.int32)
+
+        def logl_grad(self, b):
+            ta = mcp.tree_arrays(mcp.tree_from_parent(self.parent, b))
+            return oracle.logl_grad(ta, w.x, w.U, w.D, w.Uinv, w.mu, w.rates, w.pi, b)
+
+        def logl(self, b):
+            return self.logl_grad(b)[0]
+
+    prior = lambda x: (float(np.sum(np.log(10.0) - 10.0 * x)), -10.0 * np.ones_like(x))
+    rng = np.random.default_rng(5)
+    x = S.TreePosition(blv, par0)
+    lf0, g0 = plan.logl_grad(blv)
+    dev = S.NNIRefractionLeapfrog(plan, prior=prior, rng=np.random.default_rng(9))
+    host = S.NNIRefractionLeapfrog(OraclePlan(), prior=prior, rng=np.random.default_rng(9))
+    sd, sh = S.HState(x, np.zeros(N - 1), g0, lf0), S.HState(x, np.zeros(N - 1), g0, lf0)
+    for step in range(6):
+        r = rng.standard_normal(N - 1) * 3.0
+        print(f"nni leapfrog step {step}", flush=True)
+        sd = dev.one(sd.x, r, sd.g, 0.01, 1e-300)
+        sh = host.one(sh.x, r, sh.g, 0.01, 1e-300)
+        assert np.array_equal(sd.x.parent, sh.x.parent)
+        assert np.max(np.abs(np.asarray(sd.x) - np.asarray(sh.x))) <= 1e-10
+        assert rel_err(sd.lf, sh.lf) < 1e-10
+    assert dev.att_nni == host.att_nni and dev.nni == host.nni and dev.att_nni > 0
+
+    chain = S.NUTSChain(S.TreePosition(blv, par0), np.random.default_rng(17), target=0.8,
+                        tree_depth=5, molifier=False)
+    lf_ = S.NNIRefractionLeapfrog(plan, prior=prior, rng=np.random.default_rng(19))
+    draws, lfs, *_ = S.run_nuts([chain], lf_, 20, n_adapt=10)
+    assert draws[0].shape == (20, N - 1) and np.all(draws[0] > 0)
+    assert len(chain.topologies) == 20 and lf_.att_nni > 0
+    plan.set_topology(chain.x.parent)
+    ll = plan.logl(np.asarray(chain.x))
+    assert rel_err(chain.lf, ll + prior(np.asarray(chain.x))[0]) < 1e-12
