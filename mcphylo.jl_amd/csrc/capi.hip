@@ -990,7 +990,10 @@ static int rebuild_schedule(mcp_engine* e) {
     // temporary, stage 2 over the chunk sums) so no block sums millions
     e->red.clear();
     e->red2.clear();
-    const int32_t kRedChunk = 1 << 15;
+    // 4096: a stage-1 block then walks its slice in two rounds of 8
+    // independent loads per thread (32768 took 16 rounds: the straggler of
+    // the launch)
+    const int32_t kRedChunk = 1 << 12;
     int32_t ntmp = 0;
     auto add_sum = [&](int32_t dst, int32_t start, int32_t len) {
         if (len <= kRedChunk) {

@@ -121,5 +121,7 @@ def test_hybrid_schedule_single_tree_vs_oracle_and_schedule_autotune():
     best = min(rows, key=lambda r: r[2])
     assert mode == best[0] and (mode != "hybrid" or width == best[1])
     ll2, g2 = p.logl_grad(blv)
-    assert ll2 == ll and np.array_equal(g2, g)
+    # the tuned schedule may differ from hybrid width 3: logL to rounding
+    # (fused launches sum log scalers as one product per lane), gradient exact
+    assert rel_err(ll2, ll) < 1e-13 and np.array_equal(g2, g)
     p.close()
