@@ -125,3 +125,23 @@ def test_hybrid_schedule_single_tree_vs_oracle_and_schedule_autotune():
     # (fused launches sum log scalers as one product per lane), gradient exact
     assert rel_err(ll2, ll) < 1e-13 and np.array_equal(g2, g)
     p.close()
+
+
+@pytest.mark.parametrize("cpt", [1, 4])
+def test_fused_tree_too_large_for_the_lds_code_stage(cpt):
+    """600 leaves: 600 * 64 * CPT bytes of tip codes exceed the 32 KB LDS
+    stage, so the fused launch's leaves read their codes from global memory.
+    Same results as the levels schedule (gradient bit for bit) and the oracle."""
+    print(f"fused without LDS stage cpt={cpt}", flush=True)
+    w, blv, llo, go = _case(511, 600, 4, 150, 2)
+    res = {}
+    for sched in ("levels", "fused"):
+        p = mcp.Plan(w.ta.parent, w.x, 2)
+        p.set_model(mcp.Model(w.U, w.D, w.Uinv, w.mu, w.rates, w.pi))
+        p.set_tiles(4, cpt)
+        p.set_schedule(sched)
+        res[sched] = p.logl_grad(blv)
+        p.close()
+    ll, g = res["fused"]
+    assert rel_err(ll, llo) < LL_TOL and grad_err(g, go) < GRAD_TOL
+    assert rel_err(ll, res["levels"][0]) < 1e-13 and np.array_equal(g, res["levels"][1])
