@@ -1005,7 +1005,20 @@ static int rebuild_schedule(mcp_engine* e) {
             e->red.push_back(RedOp{ntmp++, start + a, std::min(kRedChunk, len - a), 1});
         e->red2.push_back(RedOp{dst, first, ntmp - first, 0});
     };
-    for (auto& it : e->items) add_sum(it.out_off, it.parts_off, it.nint * it.R * it.ntiles);
+    for (int i = 0; i < nitems; ++i) {
+        const ItemHost& it = e->items[i];
+        int v0 = -1;  // the fused launch's first post-order job, when it runs them all
+        if (e->schedule != MCP_SCHEDULE_LEVELS && h0[i] == 1)
+            for (int v = 0; v < it.N && v0 < 0; ++v)
+                if (it.nodes[v].nchild > 0 && it.height[v] == 1) v0 = v;
+        if (v0 >= 0) {
+            // the fused tasks store 0 into their log-scaler slots and each block
+            // puts its whole sum into the first job's slot: only those count
+            add_sum(it.out_off, it.parts_off + it.nodes[v0].lpart, it.R * it.ntiles);
+        } else {
+            add_sum(it.out_off, it.parts_off, it.nint * it.R * it.ntiles);
+        }
+    }
     e->n_ll_ops = (int)e->red.size();
     e->n_ll_ops2 = (int)e->red2.size();
     for (auto& it : e->items)
