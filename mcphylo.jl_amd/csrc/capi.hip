@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <queue>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -428,6 +429,60 @@ int build_dataset(const mcp_dataset_desc& d, DatasetHost& out) {
 
 void assign_partial_slots(ItemHost& it);
 
+// Pre-order slabs of a level walk: the slab of node v (depth d) is written
+// at level d - 1 and read at level d, so two ring buffers by depth parity,
+// with slots reused from depth d to depth d + 2, hold them.
+void assign_g_slots_by_depth(ItemHost& it) {
+    std::vector<int> gcount(it.max_depth + 2, 0);
+    for (int v = 0; v < it.N; ++v) {
+        NodeInfo& n = it.nodes[v];
+        n.gbuf = it.depth[v] & 1;
+        n.gslot = (n.nchild > 0 && !n.is_root) ? gcount[it.depth[v]]++ : -1;
+    }
+    it.g_slots[0] = it.g_slots[1] = 0;
+    for (int dd = 0; dd <= it.max_depth; ++dd)
+        it.g_slots[dd & 1] = std::max(it.g_slots[dd & 1], gcount[dd]);
+}
+
+// The same for a walk in arbitrary pre-order steps (step[v]: the step of
+// node v's pre-order task): the slab of v lives from its mother's step
+// (written) to its own (read).  Interval colouring in one buffer; a slot is
+// reused only by a slab written in a later step than its last reader's.
+void assign_g_slots_by_steps(ItemHost& it, const std::vector<int>& step) {
+    std::vector<int> order;
+    for (int v = 0; v < it.N; ++v)
+        if (it.nodes[v].nchild > 0 && !it.nodes[v].is_root) order.push_back(v);
+    auto start = [&](int v) { return step[it.parent[v] - 1]; };
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int a, int b) { return start(a) < start(b); });
+    std::priority_queue<std::pair<int, int>, std::vector<std::pair<int, int>>,
+                        std::greater<std::pair<int, int>>>
+        busy;  // (last step read, slot)
+    std::priority_queue<int, std::vector<int>, std::greater<int>> free_slots;
+    int nslots = 0;
+    for (int v = 0; v < it.N; ++v) {
+        it.nodes[v].gbuf = 0;
+        it.nodes[v].gslot = -1;
+    }
+    for (int v : order) {
+        while (!busy.empty() && busy.top().first < start(v)) {
+            free_slots.push(busy.top().second);
+            busy.pop();
+        }
+        int slot;
+        if (!free_slots.empty()) {
+            slot = free_slots.top();
+            free_slots.pop();
+        } else {
+            slot = nslots++;
+        }
+        it.nodes[v].gslot = slot;
+        busy.push({step[v], slot});
+    }
+    it.g_slots[0] = nslots;
+    it.g_slots[1] = 0;
+}
+
 int build_topology(ItemHost& it, const DatasetHost& d, const int32_t* parent) {
     const int N = it.N;
     if (N < 2) return fail(MCP_ERR_INVALID, "tree needs at least two nodes");
@@ -505,15 +560,11 @@ int build_topology(ItemHost& it, const DatasetHost& d, const int32_t* parent) {
             n.leaf = d.row_of.at(v + 1);
         } else {
             it.nint++;
-            if (!n.is_root) {
-                n.fslot = fslot++;
-                n.gslot = gcount[it.depth[v]]++;
-            }
+            if (!n.is_root) n.fslot = fslot++;
         }
     }
-    it.g_slots[0] = it.g_slots[1] = 0;
-    for (int dd = 0; dd <= it.max_depth; ++dd)
-        it.g_slots[dd & 1] = std::max(it.g_slots[dd & 1], gcount[dd]);
+    assign_g_slots_by_depth(it);
+    (void)gcount;
     (void)fslot;
     (void)lidx;
     assign_partial_slots(it);
@@ -565,10 +616,62 @@ static int upload(mcp_engine* e, DevBuf& b, const void* src, size_t bytes) {
 // topology change.  F/P/model arenas keep their layout (sizes do not depend
 // on topology); the pre-order ring buffers may grow, so every job record is
 // rebuilt with fresh addresses.
+// Hu's list schedule of an item's internal nodes on W processors (the
+// wavefronts of a fused block): steps of at most W nodes whose internal
+// children lie in earlier steps, the ready nodes deepest first (ties: lower
+// index).  For unit-time tasks whose precedence graph is an in-tree (a
+// post-order pass) this minimises the number of steps; run backwards it is
+// a schedule of the pre-order pass (an out-tree) with the same count.  The
+// 100-taxon tree needs 26 steps at W = 4 where its 14 height levels take 32
+// task slots per wavefront.
+static std::vector<std::vector<int>> hu_steps(const ItemHost& it, int W) {
+    std::vector<int> pending(it.N, 0);
+    for (int v = 0; v < it.N; ++v) {
+        const int p = it.parent[v] - 1;
+        if (p >= 0 && it.nodes[v].nchild > 0) pending[p]++;
+    }
+    auto deeper = [&](int a, int b) {
+        return it.depth[a] != it.depth[b] ? it.depth[a] < it.depth[b] : a > b;
+    };
+    std::priority_queue<int, std::vector<int>, decltype(deeper)> ready(deeper);
+    for (int v = 0; v < it.N; ++v)
+        if (it.nodes[v].nchild > 0 && pending[v] == 0) ready.push(v);
+    std::vector<std::vector<int>> steps;
+    while (!ready.empty()) {
+        std::vector<int> step;
+        while (!ready.empty() && (int)step.size() < W) {
+            step.push_back(ready.top());
+            ready.pop();
+        }
+        for (int v : step) {
+            const int p = it.parent[v] - 1;
+            if (p >= 0 && --pending[p] == 0) ready.push(p);
+        }
+        steps.push_back(std::move(step));
+    }
+    return steps;
+}
+
 static int rebuild_schedule(mcp_engine* e) {
     e->drop_graphs();
     e->hmc_items_dirty = true;
     const int nitems = (int)e->items.size();
+    // the fused schedule walks each item in Hu steps; pre-order slabs follow
+    // the walk's order
+    std::vector<std::vector<std::vector<int>>> hu(nitems);
+    for (int i = 0; i < nitems; ++i) {
+        ItemHost& it = e->items[i];
+        if (e->schedule == MCP_SCHEDULE_FUSED) {
+            hu[i] = hu_steps(it, e->waves);
+            std::vector<int> pstep(it.N, -1);
+            const int ns = (int)hu[i].size();
+            for (int k = 0; k < ns; ++k)
+                for (int v : hu[i][k]) pstep[v] = ns - 1 - k;  // pre-order runs the steps backwards
+            assign_g_slots_by_steps(it, pstep);
+        } else {
+            assign_g_slots_by_depth(it);
+        }
+    }
     // pre-order ring buffers
     size_t g0 = 0, g1 = 0;
     for (auto& it : e->items) {
@@ -742,6 +845,7 @@ static int rebuild_schedule(mcp_engine* e) {
     // ("tail") walks the pre-order depths >= d1 when every level from d1 down
     // is that narrow too (at least two of them).
     std::vector<int> h0(nitems), d0(nitems), d1(nitems);
+    std::vector<int> fused_first(nitems, -1);  // first post-order job of a fused-schedule item
     for (int i = 0; i < nitems; ++i) {
         const ItemHost& it = e->items[i];
         d1[i] = it.max_depth + 1;
@@ -847,16 +951,29 @@ static int rebuild_schedule(mcp_engine* e) {
         for (int S : svals)
             if ((st = add_launch(true, S, by_d[dd]))) return st;
     // model-gradient pass: every pre-order level, one k_level_mg launch per
-    // (depth, state count)
+    // (depth, state count); under the fused schedule per pre-order Hu step
+    // (the order its pre-order slabs are laid out for)
     e->adj_mg.clear();
     if (e->mg_ok) {
-        std::vector<std::vector<std::pair<int, int>>> all_d(maxd + 1);
-        for (int i = 0; i < nitems; ++i) {
-            const ItemHost& it = e->items[i];
-            for (int v = 0; v < it.N; ++v)
-                if (it.nodes[v].nchild > 0) all_d[it.depth[v]].push_back({i, v});
+        std::vector<std::vector<std::pair<int, int>>> all_d;
+        if (e->schedule == MCP_SCHEDULE_FUSED) {
+            size_t maxs = 0;
+            for (int i = 0; i < nitems; ++i) maxs = std::max(maxs, hu[i].size());
+            all_d.resize(maxs);
+            for (int i = 0; i < nitems; ++i) {
+                const int ns = (int)hu[i].size();
+                for (int k = 0; k < ns; ++k)
+                    for (int v : hu[i][k]) all_d[ns - 1 - k].push_back({i, v});
+            }
+        } else {
+            all_d.resize(maxd + 1);
+            for (int i = 0; i < nitems; ++i) {
+                const ItemHost& it = e->items[i];
+                for (int v = 0; v < it.N; ++v)
+                    if (it.nodes[v].nchild > 0) all_d[it.depth[v]].push_back({i, v});
+            }
         }
-        for (int dd = 0; dd <= maxd; ++dd)
+        for (size_t dd = 0; dd < all_d.size(); ++dd)
             for (int S : svals)
                 if ((st = add_launch(true, S, all_d[dd], &e->adj_mg))) return st;
     }
@@ -882,26 +999,51 @@ static int rebuild_schedule(mcp_engine* e) {
                 dv[it.depth[v]].push_back(v);
             }
             const size_t ks = std::find(svals.begin(), svals.end(), it.S) - svals.begin();
-            I.lf0 = (int)lf.size();
-            for (int h = h0[i]; h <= it.max_height; ++h) {
-                if (hv[h].empty()) continue;
-                lf.push_back((int32_t)jf.size());
-                for (int v : hv[h]) {
-                    jf.push_back(make_job(i, v));
-                    fbytes[ks].first += job_bytes(false, i, v);
+            if (e->schedule == MCP_SCHEDULE_FUSED) {
+                // the whole walk in one launch: Hu steps of at most `waves`
+                // nodes, backwards for the pre-order pass
+                const std::vector<std::vector<int>>& steps = hu[i];
+                I.lf0 = (int)lf.size();
+                for (const auto& step : steps) {
+                    lf.push_back((int32_t)jf.size());
+                    for (int v : step) {
+                        jf.push_back(make_job(i, v));
+                        fbytes[ks].first += job_bytes(false, i, v);
+                    }
                 }
-            }
-            I.lf1 = (int)lf.size();
-            I.la0 = (int)la.size();
-            for (int dd = 0; dd <= d0[i]; ++dd) {
-                if (dv[dd].empty()) continue;
-                la.push_back((int32_t)ja.size());
-                for (int v : dv[dd]) {
-                    ja.push_back(make_job(i, v));
-                    fbytes[ks].second += job_bytes(true, i, v);
+                I.lf1 = (int)lf.size();
+                I.la0 = (int)la.size();
+                for (auto st2 = steps.rbegin(); st2 != steps.rend(); ++st2) {
+                    la.push_back((int32_t)ja.size());
+                    for (int v : *st2) {
+                        ja.push_back(make_job(i, v));
+                        fbytes[ks].second += job_bytes(true, i, v);
+                    }
                 }
+                I.la1 = (int)la.size();
+                fused_first[i] = steps.empty() ? -1 : steps[0][0];
+            } else {
+                I.lf0 = (int)lf.size();
+                for (int h = h0[i]; h <= it.max_height; ++h) {
+                    if (hv[h].empty()) continue;
+                    lf.push_back((int32_t)jf.size());
+                    for (int v : hv[h]) {
+                        jf.push_back(make_job(i, v));
+                        fbytes[ks].first += job_bytes(false, i, v);
+                    }
+                }
+                I.lf1 = (int)lf.size();
+                I.la0 = (int)la.size();
+                for (int dd = 0; dd <= d0[i]; ++dd) {
+                    if (dv[dd].empty()) continue;
+                    la.push_back((int32_t)ja.size());
+                    for (int v : dv[dd]) {
+                        ja.push_back(make_job(i, v));
+                        fbytes[ks].second += job_bytes(true, i, v);
+                    }
+                }
+                I.la1 = (int)la.size();
             }
-            I.la1 = (int)la.size();
             const size_t k = std::find(svals.begin(), svals.end(), it.S) - svals.begin();
             per_s[k].push_back(I);
         }
@@ -1008,9 +1150,12 @@ static int rebuild_schedule(mcp_engine* e) {
     for (int i = 0; i < nitems; ++i) {
         const ItemHost& it = e->items[i];
         int v0 = -1;  // the fused launch's first post-order job, when it runs them all
-        if (e->schedule != MCP_SCHEDULE_LEVELS && h0[i] == 1)
+        if (e->schedule == MCP_SCHEDULE_FUSED) {
+            v0 = fused_first[i];
+        } else if (e->schedule == MCP_SCHEDULE_HYBRID && h0[i] == 1) {
             for (int v = 0; v < it.N && v0 < 0; ++v)
                 if (it.nodes[v].nchild > 0 && it.height[v] == 1) v0 = v;
+        }
         if (v0 >= 0) {
             // the fused tasks store 0 into their log-scaler slots and each block
             // puts its whole sum into the first job's slot: only those count
@@ -1852,6 +1997,8 @@ int mcp_set_fused_waves(mcp_batch* b, int32_t waves) {
     HIP_TRY(hipStreamSynchronize(b->stream));
     b->waves = waves;
     b->drop_graphs();  // captured fused launches hold the old block size
+    // the fused schedule groups its steps by the block's wavefront count
+    if (b->schedule == MCP_SCHEDULE_FUSED) return rebuild_schedule(b);
     return MCP_OK;
 }
 

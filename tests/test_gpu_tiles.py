@@ -145,3 +145,25 @@ def test_fused_tree_too_large_for_the_lds_code_stage(cpt):
     ll, g = res["fused"]
     assert rel_err(ll, llo) < LL_TOL and grad_err(g, go) < GRAD_TOL
     assert rel_err(ll, res["levels"][0]) < 1e-13 and np.array_equal(g, res["levels"][1])
+
+
+def test_fused_schedule_for_every_wave_count():
+    """The fused schedule groups each item's nodes into Hu steps of at most
+    one node per wavefront of a block, and lays the pre-order slabs out for
+    that order; every block size gives the oracle's results (gradient bit for
+    bit the levels schedule's)."""
+    w, blv, llo, go = _case(521, 45, 4, 400, 3)
+    p = mcp.Plan(w.ta.parent, w.x, 3)
+    p.set_model(mcp.Model(w.U, w.D, w.Uinv, w.mu, w.rates, w.pi))
+    p.set_schedule("levels")
+    ll_l, g_l = p.logl_grad(blv)
+    p.set_schedule("fused")
+    for waves in (1, 2, 3, 8, 4):
+        print(f"fused waves={waves}", flush=True)
+        p.set_fused_waves(waves)
+        ll, g = p.logl_grad(blv)
+        assert rel_err(ll, llo) < LL_TOL and grad_err(g, go) < GRAD_TOL, waves
+        assert rel_err(ll, ll_l) < 1e-13 and np.array_equal(g, g_l), waves
+        ll_m, g_m, dP, dpi = p.logl_grad_model(blv)
+        assert np.array_equal(g_m, g_l), waves
+    p.close()
