@@ -25,11 +25,13 @@
 //            the launch's task list, locates its first job once and then
 //            walks the jobs in order;
 //   k_fused  one launch per evaluation; a block owns one (item, category,
-//            tile) and walks every level of its item (see below).
+//            tile) and walks every step of its item (see below).
 //
-// Determinism: per-task sums are reduced with a fixed wavefront shuffle tree
-// plus a fixed LDS order, written to one slot per task, and summed by
-// k_reduce in a fixed order, so results are bitwise reproducible run to run.
+// Determinism: per-task sums are reduced with a fixed DPP tree per
+// wavefront, written to one slot per task (a fused block folds its log
+// scalers and writes one block sum, its wavefront sums added in a fixed
+// order), and summed by k_reduce in a fixed order, so results are bitwise
+// reproducible run to run.
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <stddef.h>
@@ -693,11 +695,14 @@ __device__ __forceinline__ void warm_record(const LevelJob& J) {
 
 // Fused schedule: every (item, category, pattern tile) is independent of all
 // other tiles through the whole evaluation, so one block walks all
-// post-order levels of its tile and then all pre-order levels, with a
-// block barrier between levels.  Within a level the block's wavefronts take
-// the level's nodes round robin (one node per wavefront, the wavefront's
-// lanes covering the tile's columns).  A tile's partials are written and
-// read back by the same block, hence on the same XCD and its L2.
+// post-order steps of its tile and then all pre-order steps, with a block
+// barrier between steps.  A step (a "level" of lvl_f / lvl_a) is a set of
+// nodes whose inputs were produced by earlier steps: the host builds Hu
+// steps of at most one node per wavefront under the fused schedule, height
+// and depth levels for the narrow top of the hybrid schedule.  Within a step
+// the block's wavefronts take its nodes round robin (the wavefront's lanes
+// covering the tile's columns).  A tile's partials are written and read back
+// by the same block, hence on the same XCD and its L2.
 template <int S, int CPT, bool GRAD>
 __global__ void __launch_bounds__(512) k_fused(const ItemSched* __restrict__ items, int nitems,
                                                const LevelJob* __restrict__ jobs_f,
