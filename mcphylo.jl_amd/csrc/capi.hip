@@ -162,6 +162,12 @@ struct NodeInfo {
 // memory.
 constexpr size_t kLdsCodesMax = 32 * 1024;
 
+// Largest LDS stage of a fused block (tip codes, then the pre-order slabs of
+// the fused walk): three such blocks fit one CU's 160 KB.  (The 100-taxon,
+// 4-state tree at 64 patterns per tile needs 6.4 KB of codes and 22 slabs of
+// 2 KB.)
+constexpr size_t kLdsFusedMax = 52 * 1024;
+
 struct DatasetHost {
     int S = 0, C = 0, Cp = 0, K = 0, nleaves = 0;
     std::vector<int32_t> leaf_nums;          // row j of codes is leaf leaf_nums[j]
@@ -218,6 +224,9 @@ struct mcp_engine {
     int waves = 4;     // wavefronts per block of the fused kernel (1..8)
     int schedule = MCP_SCHEDULE_LEVELS;
     int hybrid_width = 4;  // widest level (internal nodes) run fused under HYBRID
+    // under the fused schedule, keep an item's pre-order slabs in the block's
+    // LDS when they fit (MCP_FUSED_LDS_SLABS=0 at engine creation: global rings)
+    bool lds_slabs = true;
     int num_cu = 256;
     hipStream_t stream = nullptr;
     std::vector<DatasetHost> ds;
@@ -233,7 +242,7 @@ struct mcp_engine {
     struct FusedLaunch {
         int S, cpt, sched_begin, nitems, nblocks;
         bool tail;  // deep narrow pre-order levels, run after the per-level launches
-        size_t lds_bytes;  // tip codes staged per block (max over the launch's items)
+        size_t lds_bytes;  // LDS stage per block (max over the launch's items)
         double bytes_f, bytes_a;  // algorithmic bytes of its post-/pre-order levels
     };
     // columns per thread chosen for each state count (-1: the built default);
@@ -766,6 +775,7 @@ static int rebuild_schedule(mcp_engine* e) {
             c.dP = e->ddPT.as<double>() + it.PT_off + (size_t)n.leaf * it.R * d.K * it.S;
             c.G = nullptr;
             c.leaf = n.leaf;
+            c.gslot = -1;
         } else {
             c.F = e->dF.as<double>() + it.F_off + (size_t)n.fslot * slab;
             c.codes = nullptr;
@@ -774,6 +784,7 @@ static int rebuild_schedule(mcp_engine* e) {
             double* gb = n.gbuf ? e->dG1.as<double>() + it.G_off[1] : e->dG0.as<double>() + it.G_off[0];
             c.G = gb + (size_t)n.gslot * slab;
             c.leaf = -1;
+            c.gslot = n.gslot;
         }
         c.gpart = it.parts_off + n.gpart;
         c.mpart = it.mg_off < 0 ? -1
@@ -796,6 +807,7 @@ static int rebuild_schedule(mcp_engine* e) {
         J.lpart = it.parts_off + n.lpart;
         J.is_root = n.is_root;
         J.F = n.is_root ? nullptr : e->dF.as<double>() + it.F_off + (size_t)n.fslot * slab;
+        J.gslot = n.is_root ? -1 : n.gslot;
         if (n.is_root) {
             J.G = nullptr;
         } else {
@@ -889,18 +901,50 @@ static int rebuild_schedule(mcp_engine* e) {
     // pre-order, parent v reads its own pre-order slab unless it is the root,
     // and per child reads the partial and writes the pre-order slab (a leaf
     // child: its code)
-    auto job_bytes = [&](bool adjoint, int i, int v) -> double {
+    // (lds: the pre-order slabs stay in the fused block's LDS, so the
+    // pre-order pass reads only the partials and codes)
+    auto job_bytes = [&](bool adjoint, int i, int v, bool lds = false) -> double {
         const ItemHost& it = e->items[i];
         const DatasetHost& d = e->ds[it.dataset];
         const double w = 8.0 * it.S * it.R * d.C;
         const NodeInfo& n = it.nodes[v];
-        double b = (!n.is_root) ? w : 0.0;
+        const bool gl = adjoint && lds;
+        double b = (!n.is_root && !gl) ? w : 0.0;
         for (int q = it.child_ptr[v]; q < it.child_ptr[v + 1]; ++q) {
             const bool inner = it.nodes[it.children[q]].nchild > 0;
-            b += adjoint ? (inner ? 2.0 * w : (double)d.C) : (inner ? w : (double)d.C);
+            b += adjoint ? (inner ? (gl ? w : 2.0 * w) : (double)d.C) : (inner ? w : (double)d.C);
         }
         return b;
     };
+    // LDS stages of the fused kernel, per item: its leaves' tip-code rows of
+    // the block's tile (when they fit kLdsCodesMax and its tiles stay within
+    // the padded pattern range), then, under the fused schedule (every
+    // pre-order step of the item in the one launch), its pre-order slabs
+    // when the whole stage fits kLdsFusedMax.
+    std::vector<int> lds_rows(nitems, 0), lds_g(nitems, -1);
+    std::vector<size_t> lds_end(nitems, 0);
+    for (int i = 0; i < nitems; ++i) {
+        ItemHost& it = e->items[i];
+        const DatasetHost& d = e->ds[it.dataset];
+        const size_t tcols = 64 * (size_t)std::max(1, e->cpt(it.S));
+        const size_t cbytes = (size_t)d.nleaves * tcols;
+        if (d.nleaves > 0 && cbytes <= kLdsCodesMax && (long long)it.ntiles * tcols <= d.Cp) {
+            lds_rows[i] = d.nleaves;
+            lds_end[i] = cbytes;
+        }
+        if (e->schedule == MCP_SCHEDULE_FUSED && e->lds_slabs) {
+            const size_t goff = round_up(lds_end[i], 16);
+            const size_t gbytes = (size_t)it.g_slots[0] * it.S * tcols * sizeof(double);
+            if (goff + gbytes <= kLdsFusedMax) {
+                lds_g[i] = (int)goff;
+                lds_end[i] = goff + gbytes;
+                // mcp_get_bytes: no pre-order slab traffic for this item
+                const double w = 8.0 * it.S * it.R * d.C;
+                for (int v = 0; v < it.N; ++v)
+                    if (it.nodes[v].nchild > 0 && !it.nodes[v].is_root) e->bytes_adj -= 2.0 * w;
+            }
+        }
+    }
     auto add_launch = [&](bool adjoint, int S, const std::vector<std::pair<int, int>>& nodes,
                           std::vector<Launch>* target = nullptr) -> int {
         std::vector<LevelJob>& js = adjoint ? ja : jf;
@@ -981,7 +1025,8 @@ static int rebuild_schedule(mcp_engine* e) {
         // fused: jobs grouped by item, then level; per-item level offsets
         std::vector<int32_t> lf, la;
         std::vector<ItemSched> sched;
-        std::vector<std::vector<ItemSched>> per_s(svals.size());
+        // per state count: the items' records and the end of their LDS stages
+        std::vector<std::vector<std::pair<ItemSched, size_t>>> per_s(svals.size());
         std::vector<std::pair<double, double>> fbytes(svals.size(), {0.0, 0.0});
         for (int i = 0; i < nitems; ++i) {
             const ItemHost& it = e->items[i];
@@ -991,7 +1036,8 @@ static int rebuild_schedule(mcp_engine* e) {
             I.ntiles = it.ntiles;
             I.codes = e->dcodes.as<uint8_t>() + d.codes_off;
             I.Cp = d.Cp;
-            I.lds_leaves = d.nleaves;  // settled per launch in add_fused
+            I.lds_leaves = lds_rows[i];
+            I.lds_g = lds_g[i];
             std::vector<std::vector<int>> hv(it.max_height + 1), dv(it.max_depth + 1);
             for (int v = 0; v < it.N; ++v) {
                 if (it.nodes[v].nchild == 0) continue;
@@ -1017,7 +1063,7 @@ static int rebuild_schedule(mcp_engine* e) {
                     la.push_back((int32_t)ja.size());
                     for (int v : *st2) {
                         ja.push_back(make_job(i, v));
-                        fbytes[ks].second += job_bytes(true, i, v);
+                        fbytes[ks].second += job_bytes(true, i, v, lds_g[i] >= 0);
                     }
                 }
                 I.la1 = (int)la.size();
@@ -1045,10 +1091,10 @@ static int rebuild_schedule(mcp_engine* e) {
                 I.la1 = (int)la.size();
             }
             const size_t k = std::find(svals.begin(), svals.end(), it.S) - svals.begin();
-            per_s[k].push_back(I);
+            per_s[k].push_back({I, lds_end[i]});
         }
         // tails: the deep narrow pre-order levels of each item that has them
-        std::vector<std::vector<ItemSched>> tail_s(svals.size());
+        std::vector<std::vector<std::pair<ItemSched, size_t>>> tail_s(svals.size());
         std::vector<double> tbytes(svals.size(), 0.0);
         for (int i = 0; i < nitems; ++i) {
             const ItemHost& it = e->items[i];
@@ -1063,7 +1109,8 @@ static int rebuild_schedule(mcp_engine* e) {
             I.ntiles = it.ntiles;
             I.codes = e->dcodes.as<uint8_t>() + d.codes_off;
             I.Cp = d.Cp;
-            I.lds_leaves = d.nleaves;
+            I.lds_leaves = lds_rows[i];
+            I.lds_g = -1;
             I.lf0 = I.lf1 = 0;
             I.la0 = (int)la.size();
             for (int dd = d1[i]; dd <= it.max_depth; ++dd) {
@@ -1075,12 +1122,12 @@ static int rebuild_schedule(mcp_engine* e) {
                 }
             }
             I.la1 = (int)la.size();
-            tail_s[k].push_back(I);
+            tail_s[k].push_back({I, (size_t)lds_rows[i] * 64 * std::max(1, e->cpt(it.S))});
         }
         lf.push_back((int32_t)jf.size());
         la.push_back((int32_t)ja.size());
-        auto add_fused = [&](const std::vector<ItemSched>& items_k, int S, double bf, double ba,
-                             bool tail) -> int {
+        auto add_fused = [&](const std::vector<std::pair<ItemSched, size_t>>& items_k, int S,
+                             double bf, double ba, bool tail) -> int {
             mcp_engine::FusedLaunch F{};
             F.S = S;
             F.cpt = e->cpt(S);
@@ -1089,18 +1136,11 @@ static int rebuild_schedule(mcp_engine* e) {
             F.tail = tail;
             F.sched_begin = (int)sched.size();
             F.lds_bytes = 0;
-            // an item stages its tip codes in LDS when its rows fit kLdsCodesMax
-            // and its tiles stay within the padded pattern range
-            const int tcols = 64 * std::max(1, F.cpt);
             long long blocks = 0;
-            for (ItemSched I : items_k) {
+            for (auto [I, lend] : items_k) {
                 I.block_off = (int32_t)blocks;
                 blocks += (long long)I.R * I.ntiles;
-                const size_t bytes = (size_t)I.lds_leaves * tcols;
-                if (I.lds_leaves > 0 && bytes <= kLdsCodesMax && (long long)I.ntiles * tcols <= I.Cp)
-                    F.lds_bytes = std::max(F.lds_bytes, bytes);
-                else
-                    I.lds_leaves = 0;
+                F.lds_bytes = std::max(F.lds_bytes, lend);
                 sched.push_back(I);
             }
             if (blocks > 0x7fffffffLL) return fail(MCP_ERR_UNSUPPORTED, "too many tiles");
@@ -1258,6 +1298,7 @@ static int engine_create(mcp_engine** out, const mcp_batch_desc* d) {
     if (!e) return fail(MCP_ERR_NOMEM, "host allocation failed");
     e->device = d->device;
     e->waves = waves;
+    if (const char* v = getenv("MCP_FUSED_LDS_SLABS")) e->lds_slabs = atoi(v) != 0;
     hipDeviceProp_t prop;
     HIP_TRY(hipGetDeviceProperties(&prop, d->device));
     e->num_cu = prop.multiProcessorCount;

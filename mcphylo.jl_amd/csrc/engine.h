@@ -29,7 +29,8 @@ struct ChildRef {
     int32_t mpart;         // offset of the branch's d logL / dP partial sums (model-gradient
                            // buffer, layout [r][S*S][tile]), -1 when not laid out
     int32_t leaf;          // leaf child: its row of the dataset's code matrix; internal: -1
-    int32_t pad_;
+    int32_t gslot;         // internal child: slot of its pre-order slab (global ring, or the
+                           // fused block's LDS stage); leaf: -1
 };
 
 constexpr int kInlineChildren = 3;
@@ -44,13 +45,15 @@ struct LevelJob {
     int32_t is_root;
     int32_t rpart;          // root: offset of the d logL / d pi partial sums (model-gradient
                             // buffer, layout [S][r][tile]), else -1
+    int32_t gslot;          // slot of the own pre-order slab (non-root), else -1
+    int32_t pad_;
+    const double* pi;       // [S]
     double* F;              // output partial slab at r = 0 (non-root), else nullptr
     const double* G;        // own pre-order slab at r = 0 (non-root), else nullptr (root: pi)
     const double* weights;  // [Cp], zero in the padding
-    const double* pi;       // [S]
+    ChildRef kid[kInlineChildren];
     const double* tiptab;   // [K][S] tip vectors of the state codes
     const ChildRef* more;   // children beyond kInlineChildren
-    ChildRef kid[kInlineChildren];
 };
 
 // Inputs of the transition-matrix builder, per item.
@@ -79,7 +82,9 @@ struct ItemSched {
     int32_t lds_leaves;    // code rows each block stages in LDS (0: leaves read global codes)
     const uint8_t* codes;  // the item's dataset code matrix [leaf][Cp]
     int32_t Cp;
-    int32_t pad_;
+    int32_t lds_g;         // byte offset of the block's LDS stage of pre-order slabs (slot q:
+                           // [S][64 * CPT] doubles at lds_g + q * 512 * S * CPT); -1: the
+                           // slabs live in the global rings
 };
 
 // Reduction descriptor: out[dst] = sum(parts[start .. start+len)).

@@ -156,6 +156,14 @@ __device__ __forceinline__ void select_row(const MCP_GLOBAL double* __restrict__
 extern __shared__ uint8_t lds_codes[];
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
+// Pre-order slabs of the fused walk held in LDS (the item's lds_g >= 0):
+// slot q is [S][64 * CPT] doubles at byte lgo + q * 512 * S * CPT, so a
+// lane's accesses (lane + 64 k + 64 CPT s) are consecutive across lanes.
+template <int S, int CPT>
+__device__ __forceinline__ double* lds_gslab(int lgo, int slot) {
+    return reinterpret_cast<double*>(lds_codes + lgo) + slot * (64 * CPT * S);
+}
+
 template <int CPT>
 __device__ __forceinline__ void leaf_codes(const ChildRef& ch, int c0, int lrow, int (&code)[CPT]) {
     if (lrow) {
@@ -358,7 +366,7 @@ __device__ __forceinline__ double child_adjoint(const ChildRef& ch, const LevelJ
                                                 const double (&f)[CPT][S],
                                                 const double (&m)[CPT][S],
                                                 double (&colv)[CPT], int lrow,
-                                                const double (&dm)[CPT][S]) {
+                                                const double (&dm)[CPT][S], int lgo) {
     const MCP_GLOBAL double* wts = gptr(J.weights);
     double contrib = 0.0;
     if (ch.codes) {
@@ -416,7 +424,13 @@ __device__ __forceinline__ double child_adjoint(const ChildRef& ch, const LevelJ
             const int e = scale_exp(mx);
 #pragma unroll
             for (int s = 0; s < S; ++s) g[s] = ldexp(g[s], -e);
-            store_row<S>(Gdst + (size_t)c * S, g);
+            if (lgo >= 0) {
+                double* gl = lds_gslab<S, CPT>(lgo, ch.gslot) + (int)(threadIdx.x & 63) + 64 * k;
+#pragma unroll
+                for (int s = 0; s < S; ++s) gl[64 * CPT * s] = g[s];
+            } else {
+                store_row<S>(Gdst + (size_t)c * S, g);
+            }
             colv[k] = col;
             const double v = wts[c] * (num / col);
             contrib += c < J.C ? v : 0.0;
@@ -501,10 +515,13 @@ __device__ __forceinline__ void mg_root(const LevelJob& J, int r, int tile, int 
     }
 }
 
+// (lgo >= 0: the item's pre-order slabs live in the fused block's LDS stage
+// at byte lgo, slots J.gslot / ch.gslot)
 template <int S, int CPT, bool MG = false>
 __device__ __forceinline__ void adj_task(const LevelJob& J, int r, int tile,
                                          double* __restrict__ parts,
-                                         double* __restrict__ mparts = nullptr, int lrow = 0) {
+                                         double* __restrict__ mparts = nullptr, int lrow = 0,
+                                         int lgo = -1) {
     const int lane = (int)(threadIdx.x & 63);
     const int c0 = tile * 64 * CPT + lane;
     // the record fields the task reads, fetched as one batch of scalar loads
@@ -514,8 +531,15 @@ __device__ __forceinline__ void adj_task(const LevelJob& J, int r, int tile,
                    "s"(a.leaf), "s"(b.F), "s"(b.codes), "s"(b.P), "s"(b.dP), "s"(b.G),
                    "s"(b.gpart), "s"(b.leaf), "s"(J.G), "s"(J.pi), "s"(J.weights), "s"(J.nchild),
                    "s"(J.Cp), "s"(J.K), "s"(J.C), "s"(J.ntiles));
+    if (lgo >= 0) MCP_PIN_SCALAR("s"(J.gslot), "s"(a.gslot), "s"(b.gslot));
     double gp[CPT][S];
-    if (J.G) {
+    if (J.G && lgo >= 0) {
+        const double* src = lds_gslab<S, CPT>(lgo, J.gslot) + lane;
+#pragma unroll
+        for (int k = 0; k < CPT; ++k)
+#pragma unroll
+            for (int s = 0; s < S; ++s) gp[k][s] = src[64 * k + 64 * CPT * s];
+    } else if (J.G) {
         const MCP_GLOBAL double* src = gptr(J.G) + (size_t)r * J.Cp * S;
 #pragma unroll
         for (int k = 0; k < CPT; ++k) load_row<S>(src + (size_t)(c0 + 64 * k) * S, gp[k]);
@@ -543,15 +567,15 @@ __device__ __forceinline__ void adj_task(const LevelJob& J, int r, int tile,
             for (int s = 0; s < S; ++s) T[k][s] = gp[k][s] * mb[k][s];
         double colv[CPT];
         double ca, cb;
-        if constexpr (E) ca = child_adjoint<S, CPT, true>(a, J, r, c0, T, fa, ma, colv, lrow, da);
-        else ca = child_adjoint<S, CPT>(a, J, r, c0, T, fa, ma, colv, lrow, ma);
+        if constexpr (E) ca = child_adjoint<S, CPT, true>(a, J, r, c0, T, fa, ma, colv, lrow, da, lgo);
+        else ca = child_adjoint<S, CPT>(a, J, r, c0, T, fa, ma, colv, lrow, ma, lgo);
         if constexpr (MG) mg_branch<S, CPT>(a, J, r, tile, c0, T, fa, colv, mparts);
 #pragma unroll
         for (int k = 0; k < CPT; ++k)
 #pragma unroll
             for (int s = 0; s < S; ++s) T[k][s] = gp[k][s] * ma[k][s];
-        if constexpr (E) cb = child_adjoint<S, CPT, true>(b, J, r, c0, T, fb, mb, colv, lrow, db);
-        else cb = child_adjoint<S, CPT>(b, J, r, c0, T, fb, mb, colv, lrow, mb);
+        if constexpr (E) cb = child_adjoint<S, CPT, true>(b, J, r, c0, T, fb, mb, colv, lrow, db, lgo);
+        else cb = child_adjoint<S, CPT>(b, J, r, c0, T, fb, mb, colv, lrow, mb, lgo);
         if constexpr (MG) {
             mg_branch<S, CPT>(b, J, r, tile, c0, T, fb, colv, mparts);
             if (!J.G) {
@@ -589,7 +613,7 @@ __device__ __forceinline__ void adj_task(const LevelJob& J, int r, int tile,
         }
         child_fetch<S, CPT>(u, J, r, c0, lrow, fu, mu_);
         double colv[CPT];
-        const double cu = child_adjoint<S, CPT>(u, J, r, c0, T, fu, mu_, colv, lrow, mu_);
+        const double cu = child_adjoint<S, CPT>(u, J, r, c0, T, fu, mu_, colv, lrow, mu_, lgo);
         if constexpr (MG) mg_branch<S, CPT>(u, J, r, tile, c0, T, fu, colv, mparts);
         const double su = wave_sum(cu);
         if (lane == 0) parts[u.gpart + r * J.ntiles + tile] = su;
@@ -800,7 +824,7 @@ __global__ void __launch_bounds__(512) k_fused(const ItemSched* __restrict__ ite
             const int jn = j1 + w;
             const int jn1 = k + 1 < I.la1 ? lvl_a[k + 2] : 0;
             for (int j = lvl_a[k] + w; j < j1; j += W)
-                adj_task<S, CPT>(jobs_a[j], r, tile, parts, nullptr, lrow);
+                adj_task<S, CPT>(jobs_a[j], r, tile, parts, nullptr, lrow, I.lds_g);
             if (jn < jn1) warm_record(jobs_a[jn]);
             __syncthreads();
         }
