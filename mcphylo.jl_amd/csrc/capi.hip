@@ -152,6 +152,8 @@ struct NodeInfo {
     int32_t leaf;          // row of the dataset's tip codes (leaves), else -1
     int32_t gslot;         // pre-order slab index in ring buffer gbuf (internal non-root), else -1
     int32_t gbuf;          // 0/1: which pre-order ring buffer (depth parity)
+    int32_t fl;            // fused schedule: slot of the partial's post-order copy in the
+                           // block's LDS slab stage (internal non-root), else -1
     int32_t lpart;         // offset (in the item) of the log-scaler partial sums (internal), else -1
     int32_t gpart;         // offset (in the item) of the gradient partial sums (non-root), else -1
     int32_t is_root;
@@ -186,6 +188,8 @@ struct ItemHost {
     std::vector<int32_t> height, depth;
     int max_height = 0, max_depth = 0;
     int g_slots[2] = {0, 0};
+    int fl_slots = 0;        // LDS slots of the post-order frontier (fused schedule)
+    bool lds_stage = false;  // fused walk keeps its frontier slabs in LDS
     // model
     bool model_set = false;
     std::vector<double> U, D, Uinv, rates, pi;
@@ -492,6 +496,40 @@ void assign_g_slots_by_steps(ItemHost& it, const std::vector<int>& step) {
     it.g_slots[1] = 0;
 }
 
+// Post-order frontier of a walk in steps (step[v]: the step of node v's
+// post-order task): the partial of v is read by its mother's task, so a copy
+// lives from step[v] (written) to its mother's step (read).  Interval
+// colouring as above; slots are NodeInfo::fl.
+void assign_fl_slots_by_steps(ItemHost& it, const std::vector<int>& step) {
+    std::vector<int> order;
+    for (int v = 0; v < it.N; ++v) {
+        it.nodes[v].fl = -1;
+        if (it.nodes[v].nchild > 0 && !it.nodes[v].is_root) order.push_back(v);
+    }
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return step[a] < step[b]; });
+    std::priority_queue<std::pair<int, int>, std::vector<std::pair<int, int>>,
+                        std::greater<std::pair<int, int>>>
+        busy;  // (step read, slot)
+    std::priority_queue<int, std::vector<int>, std::greater<int>> free_slots;
+    int nslots = 0;
+    for (int v : order) {
+        while (!busy.empty() && busy.top().first < step[v]) {
+            free_slots.push(busy.top().second);
+            busy.pop();
+        }
+        int slot;
+        if (!free_slots.empty()) {
+            slot = free_slots.top();
+            free_slots.pop();
+        } else {
+            slot = nslots++;
+        }
+        it.nodes[v].fl = slot;
+        busy.push({step[it.parent[v] - 1], slot});
+    }
+    it.fl_slots = nslots;
+}
+
 int build_topology(ItemHost& it, const DatasetHost& d, const int32_t* parent) {
     const int N = it.N;
     if (N < 2) return fail(MCP_ERR_INVALID, "tree needs at least two nodes");
@@ -672,13 +710,19 @@ static int rebuild_schedule(mcp_engine* e) {
         ItemHost& it = e->items[i];
         if (e->schedule == MCP_SCHEDULE_FUSED) {
             hu[i] = hu_steps(it, e->waves);
-            std::vector<int> pstep(it.N, -1);
+            std::vector<int> pstep(it.N, -1), fstep(it.N, -1);
             const int ns = (int)hu[i].size();
             for (int k = 0; k < ns; ++k)
-                for (int v : hu[i][k]) pstep[v] = ns - 1 - k;  // pre-order runs the steps backwards
+                for (int v : hu[i][k]) {
+                    fstep[v] = k;
+                    pstep[v] = ns - 1 - k;  // pre-order runs the steps backwards
+                }
             assign_g_slots_by_steps(it, pstep);
+            assign_fl_slots_by_steps(it, fstep);
         } else {
             assign_g_slots_by_depth(it);
+            it.fl_slots = 0;
+            for (NodeInfo& n : it.nodes) n.fl = -1;
         }
     }
     // pre-order ring buffers
@@ -776,6 +820,7 @@ static int rebuild_schedule(mcp_engine* e) {
             c.G = nullptr;
             c.leaf = n.leaf;
             c.gslot = -1;
+            c.fslot = -1;
         } else {
             c.F = e->dF.as<double>() + it.F_off + (size_t)n.fslot * slab;
             c.codes = nullptr;
@@ -784,7 +829,8 @@ static int rebuild_schedule(mcp_engine* e) {
             double* gb = n.gbuf ? e->dG1.as<double>() + it.G_off[1] : e->dG0.as<double>() + it.G_off[0];
             c.G = gb + (size_t)n.gslot * slab;
             c.leaf = -1;
-            c.gslot = n.gslot;
+            c.gslot = it.lds_stage ? (int16_t)n.gslot : -1;
+            c.fslot = it.lds_stage ? (int16_t)n.fl : -1;
         }
         c.gpart = it.parts_off + n.gpart;
         c.mpart = it.mg_off < 0 ? -1
@@ -807,7 +853,8 @@ static int rebuild_schedule(mcp_engine* e) {
         J.lpart = it.parts_off + n.lpart;
         J.is_root = n.is_root;
         J.F = n.is_root ? nullptr : e->dF.as<double>() + it.F_off + (size_t)n.fslot * slab;
-        J.gslot = n.is_root ? -1 : n.gslot;
+        J.gslot = (n.is_root || !it.lds_stage) ? -1 : n.gslot;
+        J.fslot = (n.is_root || !it.lds_stage) ? -1 : n.fl;
         if (n.is_root) {
             J.G = nullptr;
         } else {
@@ -901,26 +948,28 @@ static int rebuild_schedule(mcp_engine* e) {
     // pre-order, parent v reads its own pre-order slab unless it is the root,
     // and per child reads the partial and writes the pre-order slab (a leaf
     // child: its code)
-    // (lds: the pre-order slabs stay in the fused block's LDS, so the
-    // pre-order pass reads only the partials and codes)
+    // (lds: the fused block keeps the slabs passed between its steps in LDS:
+    // the post-order pass writes the partials and reads only tip codes, the
+    // pre-order pass reads the partials and tip codes)
     auto job_bytes = [&](bool adjoint, int i, int v, bool lds = false) -> double {
         const ItemHost& it = e->items[i];
         const DatasetHost& d = e->ds[it.dataset];
         const double w = 8.0 * it.S * it.R * d.C;
         const NodeInfo& n = it.nodes[v];
-        const bool gl = adjoint && lds;
-        double b = (!n.is_root && !gl) ? w : 0.0;
+        double b = (!n.is_root && !(adjoint && lds)) ? w : 0.0;
         for (int q = it.child_ptr[v]; q < it.child_ptr[v + 1]; ++q) {
             const bool inner = it.nodes[it.children[q]].nchild > 0;
-            b += adjoint ? (inner ? (gl ? w : 2.0 * w) : (double)d.C) : (inner ? w : (double)d.C);
+            b += !inner ? (double)d.C : (adjoint ? (lds ? w : 2.0 * w) : (lds ? 0.0 : w));
         }
         return b;
     };
     // LDS stages of the fused kernel, per item: its leaves' tip-code rows of
     // the block's tile (when they fit kLdsCodesMax and its tiles stay within
-    // the padded pattern range), then, under the fused schedule (every
-    // pre-order step of the item in the one launch), its pre-order slabs
-    // when the whole stage fits kLdsFusedMax.
+    // the padded pattern range), then, under the fused schedule (every step
+    // of the item in the one launch), a slab stage when the whole stage fits
+    // kLdsFusedMax: the post-order pass keeps copies of its frontier's
+    // partials there (slots NodeInfo::fl), the pre-order pass its pre-order
+    // slabs (slots NodeInfo::gslot).
     std::vector<int> lds_rows(nitems, 0), lds_g(nitems, -1);
     std::vector<size_t> lds_end(nitems, 0);
     for (int i = 0; i < nitems; ++i) {
@@ -932,16 +981,23 @@ static int rebuild_schedule(mcp_engine* e) {
             lds_rows[i] = d.nleaves;
             lds_end[i] = cbytes;
         }
+        it.lds_stage = false;
         if (e->schedule == MCP_SCHEDULE_FUSED && e->lds_slabs) {
             const size_t goff = round_up(lds_end[i], 16);
-            const size_t gbytes = (size_t)it.g_slots[0] * it.S * tcols * sizeof(double);
+            const size_t gbytes =
+                (size_t)std::max(it.g_slots[0], it.fl_slots) * it.S * tcols * sizeof(double);
             if (goff + gbytes <= kLdsFusedMax) {
                 lds_g[i] = (int)goff;
                 lds_end[i] = goff + gbytes;
-                // mcp_get_bytes: no pre-order slab traffic for this item
+                it.lds_stage = true;
+                // mcp_get_bytes: no pre-order slab traffic, and no re-read of
+                // a partial by its mother's post-order task
                 const double w = 8.0 * it.S * it.R * d.C;
                 for (int v = 0; v < it.N; ++v)
-                    if (it.nodes[v].nchild > 0 && !it.nodes[v].is_root) e->bytes_adj -= 2.0 * w;
+                    if (it.nodes[v].nchild > 0 && !it.nodes[v].is_root) {
+                        e->bytes_adj -= 2.0 * w;
+                        e->bytes_fwd -= w;
+                    }
             }
         }
     }
@@ -1054,7 +1110,7 @@ static int rebuild_schedule(mcp_engine* e) {
                     lf.push_back((int32_t)jf.size());
                     for (int v : step) {
                         jf.push_back(make_job(i, v));
-                        fbytes[ks].first += job_bytes(false, i, v);
+                        fbytes[ks].first += job_bytes(false, i, v, lds_g[i] >= 0);
                     }
                 }
                 I.lf1 = (int)lf.size();

@@ -29,8 +29,9 @@ struct ChildRef {
     int32_t mpart;         // offset of the branch's d logL / dP partial sums (model-gradient
                            // buffer, layout [r][S*S][tile]), -1 when not laid out
     int32_t leaf;          // leaf child: its row of the dataset's code matrix; internal: -1
-    int32_t gslot;         // internal child: slot of its pre-order slab (global ring, or the
-                           // fused block's LDS stage); leaf: -1
+    int16_t gslot;         // internal child under a fused LDS slab stage: slot of its
+                           // pre-order slab, else -1
+    int16_t fslot;         // the same: slot of its partial's post-order copy, else -1
 };
 
 constexpr int kInlineChildren = 3;
@@ -45,8 +46,9 @@ struct LevelJob {
     int32_t is_root;
     int32_t rpart;          // root: offset of the d logL / d pi partial sums (model-gradient
                             // buffer, layout [S][r][tile]), else -1
-    int32_t gslot;          // slot of the own pre-order slab (non-root), else -1
-    int32_t pad_;
+    int32_t gslot;          // fused LDS slab stage: slot of the own pre-order slab (non-root),
+                            // else -1
+    int32_t fslot;          // the same: slot of the own partial's post-order copy, else -1
     const double* pi;       // [S]
     double* F;              // output partial slab at r = 0 (non-root), else nullptr
     const double* G;        // own pre-order slab at r = 0 (non-root), else nullptr (root: pi)
@@ -82,9 +84,9 @@ struct ItemSched {
     int32_t lds_leaves;    // code rows each block stages in LDS (0: leaves read global codes)
     const uint8_t* codes;  // the item's dataset code matrix [leaf][Cp]
     int32_t Cp;
-    int32_t lds_g;         // byte offset of the block's LDS stage of pre-order slabs (slot q:
-                           // [S][64 * CPT] doubles at lds_g + q * 512 * S * CPT); -1: the
-                           // slabs live in the global rings
+    int32_t lds_g;         // byte offset of the block's LDS slab stage (slot q: [S][64 * CPT]
+                           // doubles at lds_g + q * 512 * S * CPT), holding the post-order
+                           // frontier's partials, then the pre-order slabs; -1: none
 };
 
 // Reduction descriptor: out[dst] = sum(parts[start .. start+len)).

@@ -156,11 +156,13 @@ __device__ __forceinline__ void select_row(const MCP_GLOBAL double* __restrict__
 extern __shared__ uint8_t lds_codes[];
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
-// Pre-order slabs of the fused walk held in LDS (the item's lds_g >= 0):
-// slot q is [S][64 * CPT] doubles at byte lgo + q * 512 * S * CPT, so a
-// lane's accesses (lane + 64 k + 64 CPT s) are consecutive across lanes.
+// Slab stage of the fused walk in LDS (the item's lds_g >= 0): slot q is
+// [S][64 * CPT] doubles at byte lgo + q * 512 * S * CPT, so a lane's
+// accesses (lane + 64 k + 64 CPT s) are consecutive across lanes.  The
+// post-order pass keeps copies of its frontier's partials there, the
+// pre-order pass its pre-order slabs.
 template <int S, int CPT>
-__device__ __forceinline__ double* lds_gslab(int lgo, int slot) {
+__device__ __forceinline__ double* lds_slab(int lgo, int slot) {
     return reinterpret_cast<double*>(lds_codes + lgo) + slot * (64 * CPT * S);
 }
 
@@ -185,10 +187,12 @@ __device__ __forceinline__ void leaf_codes(const ChildRef& ch, int c0, int lrow,
 
 // Message each of a thread's CPT columns receives from child `ch` in
 // category r: msg[s] = sum_s1 P[s][s1] f[s1] (internal child, f loaded into
-// `f`), or the leaf's precomputed row PT[r][code] (`f` untouched).
+// `f`: from the slab stage's slot ch.fslot when lfo >= 0, else from its
+// partial slab), or the leaf's precomputed row PT[r][code] (`f` untouched).
 template <int S, int CPT>
 __device__ __forceinline__ void child_fetch(const ChildRef& ch, const LevelJob& J, int r, int c0,
-                                            int lrow, double (&f)[CPT][S], double (&m)[CPT][S]) {
+                                            int lrow, double (&f)[CPT][S], double (&m)[CPT][S],
+                                            int lfo = -1) {
     if (ch.codes) {
         const MCP_GLOBAL double* pt = gptr(ch.P) + (size_t)r * J.K * S;
         int code[CPT];
@@ -196,9 +200,17 @@ __device__ __forceinline__ void child_fetch(const ChildRef& ch, const LevelJob& 
 #pragma unroll
         for (int k = 0; k < CPT; ++k) select_row<S>(pt, code[k], m[k]);
     } else {
-        const MCP_GLOBAL double* src = gptr(ch.F) + (size_t)r * J.Cp * S;
+        if (lfo >= 0) {
+            const double* src = lds_slab<S, CPT>(lfo, ch.fslot) + (int)(threadIdx.x & 63);
 #pragma unroll
-        for (int k = 0; k < CPT; ++k) load_row<S>(src + (size_t)(c0 + 64 * k) * S, f[k]);
+            for (int k = 0; k < CPT; ++k)
+#pragma unroll
+                for (int s = 0; s < S; ++s) f[k][s] = src[64 * k + 64 * CPT * s];
+        } else {
+            const MCP_GLOBAL double* src = gptr(ch.F) + (size_t)r * J.Cp * S;
+#pragma unroll
+            for (int k = 0; k < CPT; ++k) load_row<S>(src + (size_t)(c0 + 64 * k) * S, f[k]);
+        }
         const MCP_CONST double* __restrict__ P = cptr(ch.P) + (size_t)r * (S * S);
 #pragma unroll
         for (int k = 0; k < CPT; ++k)
@@ -267,11 +279,13 @@ __device__ __forceinline__ void scale_fold(ScaleProd<CPT>& p, int k, double f) {
 
 // post-order: partial of the job's node for the thread's columns.  Its
 // log-scaler sum goes to its own slot, or (prod != nullptr) is folded into
-// *prod and the slot gets 0.
+// *prod and the slot gets 0.  (lfo >= 0: internal children are read from the
+// fused block's slab stage at byte lfo, and the partial is also copied there,
+// slot J.fslot.)
 template <int S, int CPT>
 __device__ __forceinline__ void fwd_task(const LevelJob& J, int r, int tile,
                                          double* __restrict__ parts, int lrow = 0,
-                                         ScaleProd<CPT>* prod = nullptr) {
+                                         ScaleProd<CPT>* prod = nullptr, int lfo = -1) {
     const int lane = (int)(threadIdx.x & 63);
     const int c0 = tile * 64 * CPT + lane;
     // the record fields the task reads, fetched as one batch of scalar loads
@@ -280,11 +294,12 @@ __device__ __forceinline__ void fwd_task(const LevelJob& J, int r, int tile,
     MCP_PIN_SCALAR("s"(a.F), "s"(a.codes), "s"(a.P), "s"(a.leaf), "s"(b.F), "s"(b.codes),
                    "s"(b.P), "s"(b.leaf), "s"(J.F), "s"(J.weights), "s"(J.nchild),
                    "s"(J.is_root), "s"(J.lpart), "s"(J.Cp), "s"(J.K), "s"(J.C), "s"(J.ntiles));
+    if (lfo >= 0) MCP_PIN_SCALAR("s"(J.fslot), "s"(a.fslot), "s"(b.fslot));
     double acc[CPT][S];
     if (J.nchild == 2) {
         double fa[CPT][S], ma[CPT][S], fb[CPT][S], mb[CPT][S];
-        child_fetch<S, CPT>(a, J, r, c0, lrow, fa, ma);
-        child_fetch<S, CPT>(b, J, r, c0, lrow, fb, mb);
+        child_fetch<S, CPT>(a, J, r, c0, lrow, fa, ma, lfo);
+        child_fetch<S, CPT>(b, J, r, c0, lrow, fb, mb, lfo);
 #pragma unroll
         for (int k = 0; k < CPT; ++k)
 #pragma unroll
@@ -296,7 +311,7 @@ __device__ __forceinline__ void fwd_task(const LevelJob& J, int r, int tile,
             for (int s = 0; s < S; ++s) acc[k][s] = 1.0;
         for (int q = 0; q < J.nchild; ++q) {
             double f[CPT][S], m[CPT][S];
-            child_fetch<S, CPT>(child_of(J, q), J, r, c0, lrow, f, m);
+            child_fetch<S, CPT>(child_of(J, q), J, r, c0, lrow, f, m, lfo);
 #pragma unroll
             for (int k = 0; k < CPT; ++k)
 #pragma unroll
@@ -335,6 +350,11 @@ __device__ __forceinline__ void fwd_task(const LevelJob& J, int r, int tile,
 #pragma unroll
             for (int s = 0; s < S; ++s) acc[k][s] = ldexp(acc[k][s], -e);
             store_row<S>(dst + (size_t)c * S, acc[k]);
+            if (lfo >= 0) {
+                double* fl = lds_slab<S, CPT>(lfo, J.fslot) + lane + 64 * k;
+#pragma unroll
+                for (int s = 0; s < S; ++s) fl[64 * CPT * s] = acc[k][s];
+            }
             if (prod) {
                 prod->ex[k] += e;
             } else {
@@ -425,7 +445,7 @@ __device__ __forceinline__ double child_adjoint(const ChildRef& ch, const LevelJ
 #pragma unroll
             for (int s = 0; s < S; ++s) g[s] = ldexp(g[s], -e);
             if (lgo >= 0) {
-                double* gl = lds_gslab<S, CPT>(lgo, ch.gslot) + (int)(threadIdx.x & 63) + 64 * k;
+                double* gl = lds_slab<S, CPT>(lgo, ch.gslot) + (int)(threadIdx.x & 63) + 64 * k;
 #pragma unroll
                 for (int s = 0; s < S; ++s) gl[64 * CPT * s] = g[s];
             } else {
@@ -534,7 +554,7 @@ __device__ __forceinline__ void adj_task(const LevelJob& J, int r, int tile,
     if (lgo >= 0) MCP_PIN_SCALAR("s"(J.gslot), "s"(a.gslot), "s"(b.gslot));
     double gp[CPT][S];
     if (J.G && lgo >= 0) {
-        const double* src = lds_gslab<S, CPT>(lgo, J.gslot) + lane;
+        const double* src = lds_slab<S, CPT>(lgo, J.gslot) + lane;
 #pragma unroll
         for (int k = 0; k < CPT; ++k)
 #pragma unroll
@@ -789,7 +809,7 @@ __global__ void __launch_bounds__(512) k_fused(const ItemSched* __restrict__ ite
             }
         }
         for (int j = lvl_f[k] + w; j < j1; j += W)
-            fwd_task<S, CPT>(jobs_f[j], r, tile, parts, lrow, &prod);
+            fwd_task<S, CPT>(jobs_f[j], r, tile, parts, lrow, &prod, I.lds_g);
         if (jn < jn1) warm_record(k + 1 < I.lf1 ? jobs_f[jn] : jobs_a[jn]);
         __syncthreads();
     }
