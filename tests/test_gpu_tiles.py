@@ -167,3 +167,91 @@ def test_fused_schedule_for_every_wave_count():
         ll_m, g_m, dP, dpi = p.logl_grad_model(blv)
         assert np.array_equal(g_m, g_l), waves
     p.close()
+
+
+def _fused_runs(make, monkeypatch):
+    """(logL, gradient, model gradient, algorithmic bytes) of a fused-schedule
+    engine built by make(), with the LDS slab stage on, then off."""
+    out = []
+    for off in (False, True):
+        if off:
+            monkeypatch.setenv("MCP_FUSED_LDS_SLABS", "0")
+        else:
+            monkeypatch.delenv("MCP_FUSED_LDS_SLABS", raising=False)
+        eng, blv = make()
+        eng.set_schedule("fused")
+        ll, g = eng.logl_grad(blv)
+        mg = eng.logl_grad_model(blv) if hasattr(eng, "logl_grad_model") else None
+        out.append((ll, g, mg, eng.algorithmic_bytes()))
+        eng.close()
+    monkeypatch.delenv("MCP_FUSED_LDS_SLABS", raising=False)
+    return out
+
+
+@pytest.mark.parametrize("case", ["coalescent40", "caterpillar30", "trifurcating", "cpt2"])
+def test_fused_lds_slab_stage_matches_hbm_slabs(case, monkeypatch):
+    """Under the fused schedule a block keeps the post-order frontier's
+    partials and the pre-order slabs in LDS when they fit.  The arithmetic is
+    the same as with every slab in HBM: logL, gradient and model gradient bit
+    for bit, and the algorithmic bytes drop by one partial read (post-order)
+    and one pre-order slab write and read per internal non-root node."""
+    print(f"lds slab stage {case}", flush=True)
+    if case == "trifurcating":
+        t = mcp.ParseNewick("(((A:0.2,B:0.1):0.05,(C:0.3,D:0.01):0.2):0.1,"
+                            "((E:0.6,F:0.02):0.3,G:0.1):0.2,H:0.4);")
+        ta = mcp.tree_arrays(t)
+        rng = np.random.default_rng(7)
+        C, R = 300, 3
+        x = np.zeros((4, C, ta.N), order="F")
+        for leaf in ta.leaf_nums:
+            x[rng.integers(0, 4, C), np.arange(C), leaf - 1] = 1.0
+        pi = rng.dirichlet(np.full(4, 5.0))
+        U, D, Uinv, mu = mcp.GTR(pi, rng.exponential(1.0, 6))
+        model = mcp.Model(U, D, Uinv, mu, np.asarray(mcp.discrete_gamma_rates(0.5, 0.5, R)), pi)
+        parent, blv, cpt = ta.parent, mcp.get_branchlength_vector(t), 1
+        llo, go = oracle.logl_grad(ta, x, U, D, Uinv, mu, model.rates, pi, blv)
+    else:
+        n, S, topo, cpt = {"coalescent40": (40, 4, "coalescent", 1),
+                           "caterpillar30": (30, 2, "caterpillar", 1),
+                           "cpt2": (25, 4, "yule", 2)}[case]
+        R = 4
+        w = mcp.synthetic.make_workload("custom", seed=531, n=n, S=S, C=700, R=R, topology=topo)
+        x, parent, blv = w.x, w.ta.parent, mcp.get_branchlength_vector(w.tree)
+        model = mcp.Model(w.U, w.D, w.Uinv, w.mu, w.rates, w.pi)
+        llo, go = oracle.logl_grad(w.ta, w.x, w.U, w.D, w.Uinv, w.mu, w.rates, w.pi, blv)
+    S = x.shape[0]
+
+    def make():
+        p = mcp.Plan(parent, x, R)
+        p.set_model(model)
+        p.set_tiles(S, cpt)
+        return p, blv
+
+    (ll_on, g_on, mg_on, b_on), (ll_off, g_off, mg_off, b_off) = _fused_runs(make, monkeypatch)
+    assert rel_err(ll_on, llo) < LL_TOL and grad_err(g_on, go) < GRAD_TOL
+    assert ll_on == ll_off and np.array_equal(g_on, g_off)
+    for a, b in zip(mg_on, mg_off):
+        assert np.array_equal(np.asarray(a), np.asarray(b))
+    d_fwd, d_adj = b_off[0] - b_on[0], b_off[1] - b_on[1]
+    assert d_fwd > 0 and d_adj == 2 * d_fwd
+
+
+def test_fused_lds_slab_stage_in_a_mixed_batch(monkeypatch):
+    """A batch where the slab stage fits the small trees' blocks and not the
+    large ones' (CPT 4: 8 KB slots): both kinds of item in one launch give
+    the results of the all-HBM form bit for bit."""
+    data = mcp.synthetic.make_batched_workload(n_trees=5, chains=2, n_range=(8, 200),
+                                               c_range=(300, 1500), seed=20261209)
+    ds, items, models, blv = data
+
+    def make():
+        b = mcp.Batch(ds, items)
+        for i, m in enumerate(models):
+            b.set_model(i, mcp.Model(*m))
+        for S in sorted({b.item_S[i] for i in range(len(items))}):
+            b.set_tiles(S, 4)
+        return b, blv
+
+    (ll_on, g_on, _, b_on), (ll_off, g_off, _, b_off) = _fused_runs(make, monkeypatch)
+    assert np.array_equal(ll_on, ll_off) and np.array_equal(g_on, g_off)
+    assert b_on[0] < b_off[0] and b_on[1] < b_off[1]
