@@ -746,7 +746,13 @@ __device__ __forceinline__ void warm_record(const LevelJob& J) {
 // and depth levels for the narrow top of the hybrid schedule.  Within a step
 // the block's wavefronts take its nodes round robin (the wavefront's lanes
 // covering the tile's columns).  A tile's partials are written and read back
-// by the same block, hence on the same XCD and its L2.
+// by the same block, hence on the same XCD and its L2.  When the item has a
+// slab stage in LDS (lds_g >= 0), the slabs passed from one step to a later
+// one stay on chip: a mother reads her internal children's partials from
+// their copies there, and the pre-order slabs live only there; HBM then sees
+// the partials' writes and their reads in the pre-order pass.  Every barrier
+// orders a slot's writer before its readers, and a slot's reuse after them
+// (the host's interval colouring).
 template <int S, int CPT, bool GRAD>
 __global__ void __launch_bounds__(512) k_fused(const ItemSched* __restrict__ items, int nitems,
                                                const LevelJob* __restrict__ jobs_f,
