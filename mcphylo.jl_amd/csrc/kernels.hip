@@ -722,21 +722,6 @@ __global__ void __launch_bounds__(64) k_level_mg(const LevelJob* __restrict__ jo
     }
 }
 
-// Reads one field on each 64-byte line of the record span the tasks read
-// (from C to kid[1].leaf: C, pi, kid[0].P, kid[1].P, kid[1].leaf are less
-// than 64 bytes apart), which leaves those lines in the scalar cache.
-static_assert(offsetof(LevelJob, C) <= 8 &&
-                  offsetof(LevelJob, pi) - offsetof(LevelJob, C) < 64 &&
-                  offsetof(LevelJob, kid) + offsetof(ChildRef, P) - offsetof(LevelJob, pi) < 64 &&
-                  sizeof(ChildRef) < 64 &&
-                  offsetof(ChildRef, leaf) - offsetof(ChildRef, P) < 64 &&
-                  offsetof(ChildRef, leaf) > offsetof(ChildRef, gpart) &&
-                  offsetof(ChildRef, leaf) > offsetof(ChildRef, G),
-              "warm_record touches one field per 64-byte line of the task span");
-__device__ __forceinline__ void warm_record(const LevelJob& J) {
-    MCP_PIN_SCALAR("s"(J.C), "s"(J.pi), "s"(J.kid[0].P), "s"(J.kid[1].P), "s"(J.kid[1].leaf));
-}
-
 // Fused schedule: every (item, category, pattern tile) is independent of all
 // other tiles through the whole evaluation, so one block walks all
 // post-order steps of its tile and then all pre-order steps, with a block
@@ -789,10 +774,6 @@ __global__ void __launch_bounds__(512) k_fused(const ItemSched* __restrict__ ite
         }
         __syncthreads();
     }
-    // With the gradient, before each barrier a wavefront touches the record
-    // of its first task of the next level, so that the record's loads
-    // overlap the barrier (measured: faster with the pre-order pass, slower
-    // for the post-order pass alone).
     // the post-order tasks fold their log scalers into prod; the pass ends
     // with one sum per block (see below)
     ScaleProd<CPT> prod;
@@ -803,20 +784,8 @@ __global__ void __launch_bounds__(512) k_fused(const ItemSched* __restrict__ ite
     }
     for (int k = I.lf0; k < I.lf1; ++k) {
         const int j1 = lvl_f[k + 1];
-        // the next level's job range, read with this level's bounds
-        int jn = 0, jn1 = 0;
-        if constexpr (GRAD) {
-            if (k + 1 < I.lf1) {
-                jn = j1 + w;
-                jn1 = lvl_f[k + 2];
-            } else if (I.la0 < I.la1) {
-                jn = lvl_a[I.la0] + w;
-                jn1 = lvl_a[I.la0 + 1];
-            }
-        }
         for (int j = lvl_f[k] + w; j < j1; j += W)
             fwd_task<S, CPT>(jobs_f[j], r, tile, parts, lrow, &prod, I.lds_g);
-        if (jn < jn1) warm_record(k + 1 < I.lf1 ? jobs_f[jn] : jobs_a[jn]);
         __syncthreads();
     }
     if (I.lf0 < I.lf1) {
@@ -847,11 +816,8 @@ __global__ void __launch_bounds__(512) k_fused(const ItemSched* __restrict__ ite
     if constexpr (GRAD) {
         for (int k = I.la0; k < I.la1; ++k) {
             const int j1 = lvl_a[k + 1];
-            const int jn = j1 + w;
-            const int jn1 = k + 1 < I.la1 ? lvl_a[k + 2] : 0;
             for (int j = lvl_a[k] + w; j < j1; j += W)
                 adj_task<S, CPT>(jobs_a[j], r, tile, parts, nullptr, lrow, I.lds_g);
-            if (jn < jn1) warm_record(jobs_a[jn]);
             __syncthreads();
         }
     }
